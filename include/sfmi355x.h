@@ -1,0 +1,144 @@
+/* sfmi355x.h -- C ABI of libsfmi355x.so, the gfx950 pair-histogram backend.
+ *
+ * One call computes, for every unordered point pair (i < j) of a point field, the separation
+ * distance bin and (optionally) the value bin of a structure-function operator, and accumulates
+ * per-bin sums of the operator value and exact pair counts.  This is the computation behind the
+ * reference's
+ *     calculate_structure_function(sf, x, u, distance_bins[, value_bins]; distance_metric)
+ * (src/Calculations/dispatch.jl:64-136) and its mutating form calculate_structure_function!
+ * (dispatch.jl:346-373), as reached through the backend hook
+ *     _dispatch_execution_backend(backend, shape::PointField{D}, sf, x, u, bins[, value_bins])
+ * (dispatch.jl:510-594; mutating variants dispatch.jl:377-477).  A Julia backend type plugs in
+ * there and ccalls these entry points (structurefunctions.jl_amd/julia/).
+ *
+ * Conventions (mirroring the reference):
+ *   - arrays are Julia column-major: x is (W, N), u is (D, N); element [d, i] at d + W*i;
+ *   - histograms are (n_dist) or (n_dist, n_val) column-major; sums are binary64, counts uint64;
+ *   - outputs are ACCUMULATED (+=), like calculate_structure_function! (the allocating API zeroes
+ *     first, src/Calculations/batch.jl:161-199);
+ *   - every validation the reference performs before its O(N^2) loop (shapes.jl:67-105,
+ *     serial.jl:111-120, HelperFunctions.jl:411-417, dispatch.jl:7-13) is done before any device
+ *     work; failures return a negative code and sf_last_error() describes them; nothing aborts
+ *     or throws across the ABI;
+ *   - caller pointers are not retained after return;
+ *   - a context serves one call at a time (like GPUSFWorkspace, src/Calculations/workspace.jl:44).
+ */
+#ifndef SFMI355X_H
+#define SFMI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define SF_OK 0
+#define SF_E_ARG (-1)        /* ArgumentError in the reference                 */
+#define SF_E_DIM (-2)        /* DimensionMismatch in the reference             */
+#define SF_E_COUNT (-3)      /* count width cannot hold N(N-1)/2 (serial.jl:111-120) */
+#define SF_E_DEVICE (-4)     /* HIP runtime failure                            */
+#define SF_E_UNSUPPORTED (-5)
+
+/* geometry: pair_geometry_for(distance_metric) (src/HelperFunctions.jl:390-393) */
+#define SF_GEOM_FLAT 0        /* Distances.Euclidean()        -> FlatGeometry{D}, W = D          */
+#define SF_GEOM_SPHERE_DEG 1  /* Distances.Haversine(radius)  -> SphericalGeometry, lon/lat deg  */
+#define SF_GEOM_SPHERE_RAD 2  /* Distances.SphericalAngle()   -> SphericalGeometry, radians, R=1 */
+
+/* operator kind (src/StructureFunctionTypes.jl) */
+#define SF_OP_PROJECTED 0     /* ProjectedStructureFunctionType{nl, nt}, 0 <= nl + nt <= 3 (:43-89) */
+#define SF_OP_S2 1            /* SecondOrderStructureFunctionType  ||du||^2       (:97,120)       */
+#define SF_OP_S3 2            /* ThirdOrderStructureFunctionType   du_L ||du||^2  (:106,122)      */
+#define SF_OP_T2COMP 3        /* TransverseComponentSecondOrder... ||du_t||^2/(D-1) (:137,147)    */
+#define SF_OP_L1T2COMP 4      /* LongitudinalTransverseComponentThirdOrder...     (:145,150)      */
+#define SF_OP_FULLVEC 5       /* FullVectorStructureFunctionType{nl} ||du||^nl, nl in 1..3 (:114) */
+
+/* precision */
+#define SF_PREC_FP64 0        /* plain binary64 pair arithmetic: the drop-in (counts bit-exact)  */
+#define SF_PREC_MIXED 1       /* binary32 pair arithmetic, binary64 re-binning of pairs near any
+                                 bin edge (counts bit-exact), binary64 accumulation              */
+
+typedef struct sf_ctx sf_ctx;
+
+typedef struct {
+  int32_t D;                  /* velocity components: 2 or 3                                     */
+  int32_t W;                  /* coordinate rows of x: D (flat) or 2 (sphere)                     */
+  int64_t N;                  /* points                                                           */
+  const double *x;            /* (W, N)                                                           */
+  const double *u;            /* (D, N)                                                           */
+  int32_t geometry;           /* SF_GEOM_*                                                        */
+  double radius;              /* Haversine radius (> 0); ignored unless SF_GEOM_SPHERE_DEG        */
+  int32_t op, nl, nt;         /* SF_OP_* and its exponents                                        */
+  int32_t n_dist;             /* distance bins; dist_edges has n_dist + 1 ascending entries,
+                                 bins are (e[k-1], e[k]]                                          */
+  const double *dist_edges;   /* host pointer                                                     */
+  int32_t n_val;              /* 0 => 1D histogram; else value bins of the joint histogram        */
+  const double *val_edges;    /* host pointer, n_val + 1 ascending entries (or NULL)              */
+  int32_t precision;          /* SF_PREC_*                                                        */
+  int32_t count_bits;         /* 32 or 64: width of the caller's count type (UInt32 default,
+                                 dispatch.jl:69); calls whose N(N-1)/2 exceeds it are rejected    */
+} sf_problem;
+
+/* Context: device, stream, persistent device workspace.
+ * Replaces the per-call device setup of gpu_calculate_structure_function
+ * (ext/StructureFunctionsKernelAbstractionsExt.jl:630-675) and GPUSFWorkspace
+ * (ext/gpu/workspace.jl:25-56). */
+int sf_create(int device, sf_ctx **out);
+void sf_destroy(sf_ctx *ctx);
+const char *sf_last_error(const sf_ctx *ctx);
+
+/* Host-pointer entry.  x, u, sums, counts are host memory; synchronous.
+ * Replaces gpu_calculate_structure_function[!] / gpu_calculate_structure_function_2d
+ * (src/Calculations/gpu_stubs.jl:136-184; KA ext :385-401, :804-827, :1650-1667), i.e. the
+ * PointField methods of _dispatch_execution_backend[!] (dispatch.jl:399-404, 537-544, 576-583). */
+int sf_hist(sf_ctx *ctx, const sf_problem *p, double *sums, uint64_t *counts);
+
+/* Many independent point fields ("slices", ragged N) in one persistent launch.  All slices must
+ * share D, W, geometry, radius, operator, n_dist, n_val and precision; each slice has its own
+ * edges.  Slice s accumulates into sums/counts[s * ncell .. (s+1) * ncell), ncell = n_dist * max(n_val,1).
+ * Replaces gpu_calculate_structure_function_batch[!] (gpu_stubs.jl:191-213), generalised from
+ * the reference's equal-N batches (batch_leading.jl:43-63) to ragged N. */
+int sf_hist_batch(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *sums,
+                  uint64_t *counts);
+
+/* Device-pointer entries: x, u (inside each sf_problem), sums and counts are device memory on
+ * the context's device; edges stay host pointers.  Work is enqueued on `stream` (a hipStream_t;
+ * NULL = the context's own stream) and the call returns after enqueueing.  Device buffers must
+ * stay alive until the stream reaches that point.  Counts/sums are accumulated atomically.
+ * (The reference has no device-pointer entry; its GPU path stages host arrays each call,
+ * KA ext :309-355.  This serves callers whose fields already live in HBM.) */
+int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d_counts,
+                   void *stream);
+int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
+                         uint64_t *d_counts, void *stream);
+
+/* Diagnostics of the last successful call on this context (for tests and benchmarks):
+ * out[0] = pairs visited, out[1] = pairs re-binned in binary64 by the mixed kernel,
+ * out[2] = work items, out[3] = workgroups launched, out[4] = histogram kernel time (ns),
+ * out[5] = re-bin kernel time (ns), out[6] = device time from the histogram launch to the
+ * final merge (ns),
+ * out[7] = threads per workgroup.  Times come from HIP events recorded on the call's stream
+ * around the launches; reading them waits for that stream.  Returns 0. */
+int sf_last_stats(sf_ctx *ctx, int64_t out[8]);
+
+/* Validation only, no device work and no context needed: returns the code sf_hist_batch would
+ * return for these slices before launching, and writes the message into msg (if not NULL).
+ * Mirrors the reference's pre-loop checks (shapes.jl:67-105, serial.jl:111-120,
+ * HelperFunctions.jl:411-417). */
+int sf_check(int32_t n_slices, const sf_problem *slices, char *msg, int32_t msg_len);
+
+/* Device self-test of the deterministic binary64 functions the bit-exact path depends on:
+ * fn 0: sincosd(a) -> out[2k], out[2k+1];  fn 1: sincos(a) -> out[2k], out[2k+1];
+ * fn 2: atan2(a, b) -> out[k];  fn 3: sqrt(a) -> out[k];  fn 4: a / b -> out[k].
+ * Host pointers; n values.  (Diagnostic; no reference counterpart.) */
+int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a, const double *b, double *out);
+
+/* Kernel tuning override (autotuner hook): block size, i-tile rows per block, j-chunk length per
+ * work item.  Zero keeps the built-in choice.  Applies to subsequent calls on this context. */
+int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFMI355X_H */

@@ -1,0 +1,824 @@
+// sf_api.hip -- host side of libsfmi355x.so: validation, bin tables, work items, launches.
+//
+// Validation mirrors what the reference checks before its O(N^2) loop:
+//   _validate_array_shape          src/Calculations/shapes.jl:67-105  (D in {2,3}, W, N match)
+//   _assert_counts_representable   src/Calculations/serial.jl:111-120 (N(N-1)/2 fits the count type)
+//   pair_geometry_for              src/HelperFunctions.jl:390-417     (known metric only)
+//   operator validity              src/StructureFunctionTypes.jl:43-151
+// plus the preconditions the reference assumes silently: edges sorted (searchsortedfirst on an
+// unsorted vector is meaningless) and, for the flat squared plan (BinEdges.jl:440), non-negative.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sfmi355x.h"
+#include "sf_internal.h"
+
+namespace sfk {
+// kernel launchers (sf_kernels.hip)
+hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *prep_slices,
+                       int n_slices, int64_t max_n, double *rec64, float *rec32, hipStream_t st);
+hipError_t launch_hist(int G, int D, bool J, bool mixed, const Params &P, int grid, int block,
+                       size_t smem, hipStream_t st);
+hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st);
+hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                        unsigned long long *dst_c, int64_t n, hipStream_t st);
+hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
+hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
+                            hipStream_t st);
+int hist_occupancy(int G, int D, bool J, bool mixed, int block, size_t smem);
+struct PrepSliceHost {
+  const double *x, *u;
+  int64_t N, pt_off;
+  double xscale;
+};
+}  // namespace sfk
+
+using namespace sfk;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t want = std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+bool jl_isless(double a, double b) {
+  if (a != a) return false;
+  if (b != b) return true;
+  if (a < b) return true;
+  if (a == b) return signbit(a) && !signbit(b);
+  return false;
+}
+
+int op_order(const sf_problem &p) {
+  switch (p.op) {
+    case SF_OP_PROJECTED: return p.nl + p.nt;
+    case SF_OP_S2: return 2;
+    case SF_OP_S3: return 3;
+    case SF_OP_T2COMP: return 2;
+    case SF_OP_L1T2COMP: return 3;
+    default: return p.nl;
+  }
+}
+
+float round_down_f(long double x) {
+  float f = (float)x;
+  if ((long double)f > x) f = nextafterf(f, -INFINITY);
+  return f;
+}
+float round_up_f(long double x) {
+  float f = (float)x;
+  if ((long double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+// Cell partition of the positive key axis: binades [2^elo, 2^ehi) cut into 2^M cells each.
+struct Partition {
+  int M;
+  int elo, ehi;
+  int ncell() const { return (ehi - elo) << M; }
+  // real lower bound of cell c (c = 0 also stands for everything below)
+  long double low(int c) const {
+    const int e = elo + (c >> M);
+    const int m = c & ((1 << M) - 1);
+    return ldexpl(1.0L + (long double)m / (long double)(1 << M), e);
+  }
+};
+
+Partition make_partition(const std::vector<long double> &keys, int cap_cells, int Mmax) {
+  long double mn = INFINITY, mx = 0;
+  for (long double k : keys)
+    if (k > 0 && k < INFINITY) {
+      mn = std::min(mn, k);
+      mx = std::max(mx, k);
+    }
+  Partition P;
+  if (!(mn < INFINITY)) {  // no positive finite key
+    P.elo = 0;
+    P.ehi = 1;
+    P.M = 0;
+    return P;
+  }
+  int e0, e1;
+  frexpl(mn, &e0);  // mn in [2^(e0-1), 2^e0)
+  frexpl(mx, &e1);
+  P.elo = e0 - 2;   // one spare binade below the smallest key
+  P.ehi = e1 + 1;   // and above the largest
+  P.elo = std::max(P.elo, -1020);
+  P.ehi = std::min(P.ehi, 1020);
+  P.M = 0;
+  for (int M = Mmax; M >= 0; --M) {
+    if (((P.ehi - P.elo) << M) <= cap_cells) {
+      P.M = M;
+      break;
+    }
+  }
+  return P;
+}
+
+}  // namespace
+
+struct sf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  size_t max_smem = 0;
+  std::string err;
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_pending = false;
+  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0;
+  // device buffers
+  DevBuf<double> d_x, d_u;
+  DevBuf<double> rec64;
+  DevBuf<float> rec32;
+  DevBuf<Slice> slices;
+  DevBuf<Item> items;
+  DevBuf<Cell> cells;
+  DevBuf<double> dedges, vedges;
+  DevBuf<float> vedges32;
+  DevBuf<double> acc_s;
+  DevBuf<unsigned long long> acc_c;
+  DevBuf<int> counter;
+  DevBuf<unsigned long long> pcount;
+  DevBuf<Pending> pending;
+  DevBuf<PrepSliceHost> prep;
+
+  int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+};
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ctx->fail(SF_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+namespace {
+
+int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
+  if (!p) return ctx->fail(SF_E_ARG, "slice %d: null problem", idx);
+  if (p->D != 2 && p->D != 3)
+    return ctx->fail(SF_E_DIM, "expected spatial/velocity dimension D=2 or D=3 on axis 1; got D=%d", p->D);
+  if (p->geometry == SF_GEOM_FLAT) {
+    if (p->W != p->D)
+      return ctx->fail(SF_E_DIM,
+                       "this geometry locates a point with %d coordinate(s) on axis 1 of x, but got "
+                       "size(x,1)=%d (velocity dimension D=%d from size(u,1))", p->D, p->W, p->D);
+  } else if (p->geometry == SF_GEOM_SPHERE_DEG || p->geometry == SF_GEOM_SPHERE_RAD) {
+    if (p->W != 2)
+      return ctx->fail(SF_E_DIM,
+                       "this geometry locates a point with 2 coordinate(s) on axis 1 of x, but got "
+                       "size(x,1)=%d (velocity dimension D=%d from size(u,1))", p->W, p->D);
+    if (p->geometry == SF_GEOM_SPHERE_DEG && !(p->radius > 0.0 && p->radius < INFINITY))
+      return ctx->fail(SF_E_ARG, "Haversine radius must be positive and finite; got %g", p->radius);
+  } else {
+    return ctx->fail(SF_E_ARG, "no pair geometry is defined for geometry code %d", p->geometry);
+  }
+  if (p->N < 0) return ctx->fail(SF_E_DIM, "N must be non-negative; got %lld", (long long)p->N);
+  if (p->N >= (1LL << 31) - 4096)
+    return ctx->fail(SF_E_UNSUPPORTED, "N=%lld exceeds this backend's 2^31 point limit", (long long)p->N);
+  if (p->N > 0 && (!p->x || !p->u)) return ctx->fail(SF_E_ARG, "slice %d: null x or u", idx);
+  switch (p->op) {
+    case SF_OP_PROJECTED:
+      if (p->nl < 0 || p->nt < 0 || p->nl + p->nt > 3)
+        return ctx->fail(SF_E_UNSUPPORTED,
+                         "ProjectedStructureFunctionType{%d,%d}: this backend supports NL,NT >= 0, NL+NT <= 3",
+                         p->nl, p->nt);
+      break;
+    case SF_OP_S2: case SF_OP_S3: case SF_OP_T2COMP: case SF_OP_L1T2COMP:
+      break;
+    case SF_OP_FULLVEC:
+      if (p->nl < 1 || p->nl > 3)
+        return ctx->fail(SF_E_UNSUPPORTED, "FullVectorStructureFunctionType{%d}: NF must be 1..3", p->nl);
+      break;
+    default:
+      return ctx->fail(SF_E_ARG, "unknown operator code %d (derived operators are not pairwise)", p->op);
+  }
+  if (p->n_dist < 1 || !p->dist_edges)
+    return ctx->fail(SF_E_ARG, "distance_bins must have at least 2 edges");
+  for (int k = 0; k <= p->n_dist; ++k) {
+    const double e = p->dist_edges[k];
+    if (e != e) return ctx->fail(SF_E_ARG, "distance_bins contains NaN");
+    if (k > 0 && jl_isless(e, p->dist_edges[k - 1]))
+      return ctx->fail(SF_E_ARG, "distance_bins must be sorted ascending");
+  }
+  if (p->geometry == SF_GEOM_FLAT && p->dist_edges[0] < 0.0)
+    return ctx->fail(SF_E_ARG, "Euclidean distance_bins must be non-negative (squared-distance plan)");
+  if (p->n_val < 0) return ctx->fail(SF_E_ARG, "n_val must be >= 0");
+  if (p->n_val > 0) {
+    if (!p->val_edges) return ctx->fail(SF_E_ARG, "value_bins missing");
+    for (int k = 0; k <= p->n_val; ++k) {
+      const double e = p->val_edges[k];
+      if (e != e) return ctx->fail(SF_E_ARG, "value_bins contains NaN");
+      if (k > 0 && jl_isless(e, p->val_edges[k - 1]))
+        return ctx->fail(SF_E_ARG, "value_bins must be sorted ascending");
+    }
+  }
+  if (p->precision != SF_PREC_FP64 && p->precision != SF_PREC_MIXED)
+    return ctx->fail(SF_E_ARG, "precision must be SF_PREC_FP64 or SF_PREC_MIXED");
+  if (p->count_bits != 32 && p->count_bits != 64)
+    return ctx->fail(SF_E_ARG, "count_bits must be 32 or 64");
+  const unsigned __int128 pairs = (unsigned __int128)p->N * (unsigned __int128)(p->N > 0 ? p->N - 1 : 0) / 2;
+  const unsigned __int128 cmax = p->count_bits == 32 ? 0xffffffffull : 0xffffffffffffffffull;
+  if (pairs > cmax)
+    return ctx->fail(SF_E_COUNT,
+                     "count_eltype=UInt%d cannot represent the worst-case pair count for N=%lld; "
+                     "pass count_eltype=UInt64 or Int64.", p->count_bits, (long long)p->N);
+  if (host_ptrs && p->geometry == SF_GEOM_SPHERE_RAD) {
+    for (int64_t k = 0; k < 2 * p->N; ++k)
+      if (fabs(p->x[k]) > 1.6e6)
+        return ctx->fail(SF_E_UNSUPPORTED, "SphericalAngle coordinates beyond |1.6e6| rad are not supported");
+  }
+  return SF_OK;
+}
+
+int validate_batch(sf_ctx *ctx, int n, const sf_problem *s, bool host_ptrs) {
+  if (n < 0) return ctx->fail(SF_E_ARG, "n_slices must be >= 0");
+  if (n > 0 && !s) return ctx->fail(SF_E_ARG, "null slices");
+  for (int k = 0; k < n; ++k) {
+    int rc = validate_one(ctx, &s[k], host_ptrs, k);
+    if (rc) return rc;
+    if (k > 0) {
+      const sf_problem &a = s[0], &b = s[k];
+      if (a.D != b.D || a.W != b.W || a.geometry != b.geometry || a.op != b.op || a.nl != b.nl ||
+          a.nt != b.nt || a.n_dist != b.n_dist || a.n_val != b.n_val || a.precision != b.precision ||
+          (a.geometry == SF_GEOM_SPHERE_DEG && a.radius != b.radius))
+        return ctx->fail(SF_E_ARG, "slice %d: all slices must share D, geometry, operator, bin counts and precision", k);
+    }
+  }
+  return SF_OK;
+}
+
+// Host-side tables for one slice.
+struct SliceTables {
+  std::vector<Cell> tab64, tab32;
+  std::vector<double> dedges;  // exact keys
+  Slice desc{};
+};
+
+void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
+  const int nE = p.n_dist + 1;
+  const bool flat = p.geometry == SF_GEOM_FLAT;
+  const long double R = flat ? 1.0L : (p.geometry == SF_GEOM_SPHERE_DEG ? (long double)p.radius : 1.0L);
+  // ---- binary64 exact keys and table
+  T.dedges.resize(nE);
+  std::vector<long double> k64(nE);
+  for (int k = 0; k < nE; ++k) {
+    const double e = p.dist_edges[k];
+    T.dedges[k] = flat ? e * e : (e == 0.0 ? 0.0 : e);  // -0.0 -> +0.0 (keys are > 0)
+    k64[k] = T.dedges[k];
+  }
+  Partition P64 = make_partition(k64, 1024, 6);
+  const int n64 = P64.ncell();
+  T.tab64.resize(n64);
+  for (int c = 0; c < n64; ++c) {
+    const long double lowl = (c == 0) ? 0.0L : P64.low(c);
+    const double low = (double)lowl;  // cell bounds are exact binary64 values
+    int b = 0;
+    if (c == 0) {
+      // keys below the window: only non-positive edges are below them
+      for (int k = 0; k < nE; ++k) if (T.dedges[k] <= 0.0) b = k + 1;
+    } else {
+      for (int k = 0; k < nE; ++k) if (T.dedges[k] < low) b = k + 1;
+    }
+    T.tab64[c].bx = b;
+    T.tab64[c].lo = INFINITY;
+    T.tab64[c].hi = INFINITY;
+    T.tab64[c].blo = 0;
+  }
+  // ---- binary32 keys: flat r^2 = e^2; sphere chord^2 = (2 sin(e / 2R))^2
+  std::vector<long double> C(nE);
+  for (int k = 0; k < nE; ++k) {
+    const long double e = p.dist_edges[k];
+    if (flat) {
+      C[k] = e * e;
+    } else if (e <= 0) {
+      C[k] = e;
+    } else {
+      const long double th = std::min(e / R, (long double)M_PIl);
+      const long double c = 2.0L * sinl(0.5L * th);
+      C[k] = c * c;
+    }
+  }
+  // margins: relative 2^-18 (64 ulp) plus an absolute term for the hi/lo split of the inputs
+  const long double drel = ldexpl(1.0L, -18);
+  const long double e1 = flat ? ldexpl(1.0L, -44) * std::max((long double)xmax, 1e-300L) : ldexpl(1.0L, -44);
+  std::vector<float> lo(nE), hi(nE);
+  for (int k = 0; k < nE; ++k) {
+    if (C[k] == INFINITY) { lo[k] = INFINITY; hi[k] = INFINITY; continue; }
+    const long double a = 2.0L * sqrtl(fabsl(C[k])) * e1 + e1 * e1 + 1e-30L;
+    lo[k] = round_down_f(C[k] - fabsl(C[k]) * drel - a);
+    hi[k] = round_up_f(C[k] + fabsl(C[k]) * drel + a);
+  }
+  std::vector<long double> ckeys(C.begin(), C.end());
+  Partition P32 = make_partition(ckeys, 1024, 6);
+  // keep every cell's lower bound representable in binary32
+  P32.elo = std::max(P32.elo, -125);
+  P32.ehi = std::min(P32.ehi, 127);
+  if (P32.ehi <= P32.elo) P32.ehi = P32.elo + 1;
+  const int n32 = P32.ncell();
+  T.tab32.resize(n32);
+  for (int c = 0; c < n32; ++c) {
+    const float Fc = (c == 0) ? -INFINITY : (float)P32.low(c);
+    const float Fn = (c + 1 >= n32) ? INFINITY : (float)P32.low(c + 1);
+    int nint = 0, which = -1, below = 0;
+    for (int k = 0; k < nE; ++k) {
+      if (hi[k] < Fc) { ++below; continue; }
+      if (lo[k] >= Fn) continue;
+      ++nint;
+      which = k;
+    }
+    Cell &cl = T.tab32[c];
+    cl.bx = 0;
+    cl.blo = below;
+    if (nint == 0) {
+      cl.lo = INFINITY;
+      cl.hi = INFINITY;
+    } else if (nint == 1) {
+      cl.lo = lo[which];
+      cl.hi = hi[which];
+    } else {
+      cl.lo = -INFINITY;
+      cl.hi = INFINITY;
+    }
+  }
+  Slice &s = T.desc;
+  s.M64 = P64.M;
+  s.ncell64 = n64;
+  s.base64 = ((int64_t)(P64.elo + 1023) << P64.M);  // cell 0 starts at 2^elo
+  s.M32 = P32.M;
+  s.ncell32 = n32;
+  s.base32 = ((P32.elo + 127) << P32.M);
+  // value axis
+  s.v_uniform = 0;
+  s.v_negzero = 0;
+  s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0;
+  if (p.n_val > 0) {
+    const int nv = p.n_val;
+    const double *E = p.val_edges;
+    for (int k = 0; k <= nv; ++k) if (E[k] == 0.0 && signbit(E[k])) s.v_negzero++;
+    const double v0 = E[0], v1 = E[nv];
+    if (v0 > -INFINITY && v1 < INFINITY && v1 > v0) {
+      const long double w = ((long double)v1 - v0) / nv;
+      long double dev = 0;
+      for (int k = 0; k <= nv; ++k) dev = std::max(dev, fabsl(((long double)E[k] - v0) / w - k));
+      if (dev < 0.25L) {
+        s.v_uniform = 1;
+        s.v0 = v0;
+        s.vinv = (double)(1.0L / w);
+        s.v0f = (float)v0;
+        s.vinvf = (float)(1.0L / w);
+        // slack in t units: edge non-uniformity + rounding of v0 and 1/w in binary32
+        const long double u = ldexpl(1.0L, -24);
+        s.vslack = round_up_f(dev + 4.0L * u * (1.0L + fabsl((long double)v0 / w)) + 4.0L * u * nv + 1e-6L);
+      }
+    }
+    if (!s.v_uniform) s.vslack = (float)ldexpl(1.0L, -21);  // relative slack for binary32 edges
+  }
+}
+
+int64_t pairs_of(int64_t n) { return n > 1 ? n * (n - 1) / 2 : 0; }
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------------------------
+extern "C" int sf_create(int device, sf_ctx **out) {
+  if (!out) return SF_E_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SF_E_DEVICE;
+  if (device < 0 || device >= ndev) return SF_E_ARG;
+  sf_ctx *ctx = new sf_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete ctx; return SF_E_DEVICE; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return SF_E_DEVICE; }
+  ctx->n_cu = prop.multiProcessorCount;
+  ctx->max_smem = prop.sharedMemPerBlock;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+    ctx->max_smem = (size_t)v;
+  if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return SF_E_DEVICE; }
+  *out = ctx;
+  return SF_OK;
+}
+
+extern "C" void sf_destroy(sf_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->d_x.release(); ctx->d_u.release(); ctx->rec64.release(); ctx->rec32.release();
+  ctx->slices.release(); ctx->items.release(); ctx->cells.release(); ctx->dedges.release();
+  ctx->vedges.release(); ctx->vedges32.release(); ctx->acc_s.release(); ctx->acc_c.release();
+  ctx->counter.release(); ctx->pcount.release(); ctx->pending.release(); ctx->prep.release();
+  for (int k = 0; k < 4; ++k)
+    if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char *sf_last_error(const sf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int sf_last_stats(sf_ctx *ctx, int64_t out[8]) {
+  if (!ctx || !out) return SF_E_ARG;
+  if (ctx->ev_pending) {
+    ctx->ev_pending = false;
+    float t01 = 0, t12 = 0, t03 = 0;
+    if (hipEventSynchronize(ctx->ev[3]) == hipSuccess) {
+      (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
+      (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
+      (void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
+    }
+    ctx->stats[4] = (int64_t)(t01 * 1e6);
+    ctx->stats[5] = (int64_t)(t12 * 1e6);
+    ctx->stats[6] = (int64_t)(t03 * 1e6);
+  }
+  for (int k = 0; k < 8; ++k) out[k] = ctx->stats[k];
+  return SF_OK;
+}
+
+extern "C" int sf_check(int32_t n_slices, const sf_problem *slices, char *msg, int32_t msg_len) {
+  sf_ctx tmp;  // carries the message only; no device state is touched
+  const int rc = validate_batch(&tmp, n_slices, slices, true);
+  if (msg && msg_len > 0) {
+    strncpy(msg, tmp.err.c_str(), (size_t)msg_len - 1);
+    msg[msg_len - 1] = 0;
+  }
+  return rc;
+}
+
+extern "C" int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a, const double *b,
+                            double *out) {
+  if (!ctx) return SF_E_ARG;
+  if (fn < 0 || fn > 4 || n < 0 || (n > 0 && (!a || !out))) return ctx->fail(SF_E_ARG, "bad arguments");
+  if ((fn == 2 || fn == 4) && n > 0 && !b) return ctx->fail(SF_E_ARG, "fn needs b");
+  if (n == 0) return SF_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nout = (fn <= 1) ? 2 : 1;
+  DevBuf<double> da, db, dout;
+  HIPCHK(da.ensure((size_t)n));
+  HIPCHK(db.ensure((size_t)n));
+  HIPCHK(dout.ensure((size_t)n * nout));
+  hipStream_t st = ctx->stream;
+  HIPCHK(hipMemcpyAsync(da.p, a, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  if (b) HIPCHK(hipMemcpyAsync(db.p, b, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_eval_math(fn, n, da.p, db.p, dout.p, st));
+  HIPCHK(hipMemcpyAsync(out, dout.p, sizeof(double) * n * nout, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  da.release(); db.release(); dout.release();
+  return SF_OK;
+}
+
+extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu) {
+  if (!ctx) return SF_E_ARG;
+  if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512)
+    return ctx->fail(SF_E_ARG, "block must be 0, 64, 128, 256 or 512");
+  if (j_chunk < 0 || blocks_per_cu < 0) return ctx->fail(SF_E_ARG, "tuning values must be >= 0");
+  ctx->tune_block = block;
+  ctx->tune_jchunk = j_chunk;
+  ctx->tune_bpc = blocks_per_cu;
+  return SF_OK;
+}
+
+namespace {
+
+// Core: slices' x/u are device pointers (for the host API they were uploaded first).  Accumulates
+// into device outputs out_s/out_c (n_slices * ncell each).
+int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
+              const double *const *du, const double *xmax, double *out_s,
+              unsigned long long *out_c, hipStream_t st) {
+  const sf_problem &p0 = sl[0];
+  const bool flat = p0.geometry == SF_GEOM_FLAT;
+  const int G = flat ? kFlat : kSphere;
+  const int D = p0.D;
+  const bool J = p0.n_val > 0;
+  const bool mixed = p0.precision == SF_PREC_MIXED;
+  const int ncell = p0.n_dist * (J ? p0.n_val : 1);
+  const int S64 = flat ? (D == 2 ? 4 : 6) : 12;
+  const int F32 = (flat && D == 2) ? 8 : 12;
+
+  // ---- records
+  int64_t total_pts = 0, max_n = 0, total_pairs = 0;
+  std::vector<PrepSliceHost> prep(n);
+  for (int s = 0; s < n; ++s) {
+    prep[s].x = dx[s];
+    prep[s].u = du[s];
+    prep[s].N = sl[s].N;
+    prep[s].pt_off = total_pts;
+    prep[s].xscale = 0;
+    total_pts += sl[s].N;
+    max_n = std::max<int64_t>(max_n, sl[s].N);
+    total_pairs += pairs_of(sl[s].N);
+  }
+  for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
+  ctx->stats[0] = total_pairs;
+  ctx->ev_pending = false;
+  if (total_pairs == 0) return SF_OK;
+  for (int k = 0; k < 4; ++k)
+    if (!ctx->ev[k]) HIPCHK(hipEventCreate(&ctx->ev[k]));
+  HIPCHK(ctx->rec64.ensure((size_t)total_pts * S64 + 16));
+  if (mixed) HIPCHK(ctx->rec32.ensure((size_t)total_pts * F32 + 16));
+  HIPCHK(ctx->prep.ensure(n));
+  HIPCHK(hipMemcpyAsync(ctx->prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_prep(G, D, mixed, p0.geometry, ctx->prep.p, n, max_n, ctx->rec64.p, ctx->rec32.p, st));
+
+  // ---- tables
+  std::vector<Slice> sdesc(n);
+  std::vector<Cell> cells;
+  std::vector<double> dedges, vedges;
+  std::vector<float> vedges32;
+  int ncell_max = 1;
+  for (int s = 0; s < n; ++s) {
+    SliceTables T;
+    build_tables(sl[s], xmax[s], T);
+    Slice d = T.desc;
+    d.N = sl[s].N;
+    d.pt_off = prep[s].pt_off;
+    d.out_off = (int64_t)s * ncell;
+    d.tab_off = (int32_t)cells.size();
+    cells.insert(cells.end(), T.tab64.begin(), T.tab64.end());
+    d.tab32_off = (int32_t)cells.size();
+    cells.insert(cells.end(), T.tab32.begin(), T.tab32.end());
+    d.dedge_off = (int32_t)dedges.size();
+    dedges.insert(dedges.end(), T.dedges.begin(), T.dedges.end());
+    d.vedge_off = (int32_t)vedges.size();
+    if (J) {
+      for (int k = 0; k <= p0.n_val; ++k) {
+        vedges.push_back(sl[s].val_edges[k]);
+        vedges32.push_back((float)sl[s].val_edges[k]);
+      }
+    }
+    ncell_max = std::max(ncell_max, std::max(d.ncell64, d.ncell32));
+    sdesc[s] = d;
+  }
+  if (vedges.empty()) { vedges.push_back(0.0); vedges32.push_back(0.0f); }
+
+  // ---- launch geometry
+  int block = ctx->tune_block ? ctx->tune_block : 256;
+  const int stage_bytes = mixed ? block * F32 * 4 : block * S64 * 8;
+  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes);
+  if ((size_t)L.total > ctx->max_smem)
+    return ctx->fail(SF_E_UNSUPPORTED,
+                     "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
+                     ncell, L.total, ctx->max_smem);
+  int bpc = hist_occupancy(G, D, J, mixed, block, (size_t)L.total);
+  if (bpc < 1) bpc = 1;
+  if (ctx->tune_bpc) bpc = std::min(bpc, ctx->tune_bpc);
+  const int max_blocks = bpc * ctx->n_cu;
+
+  // ---- work items: (slice, i-tile, j-chunk); slices in descending N, large chunks first
+  int jchunk = ctx->tune_jchunk;
+  if (jchunk <= 0) {
+    const double target_items = 16.0 * max_blocks;
+    double per_item = (double)total_pairs / target_items;  // pairs per item
+    int jc = (int)(per_item / block);
+    jc = std::max(jc, block);
+    jc = std::min(jc, 64 * block);
+    jchunk = (jc + block - 1) / block * block;
+  }
+  std::vector<int> order(n);
+  for (int s = 0; s < n; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sl[a].N > sl[b].N; });
+  std::vector<Item> items;
+  for (int s : order) {
+    const int64_t N = sl[s].N;
+    for (int64_t i0 = 0; i0 < N - 1; i0 += block) {
+      for (int64_t j0 = i0; j0 < N; j0 += jchunk) {
+        Item it;
+        it.slice = s;
+        it.i0 = (int32_t)i0;
+        it.j0 = (int32_t)j0;
+        it.j1 = (int32_t)std::min<int64_t>(N, j0 + jchunk);
+        items.push_back(it);
+      }
+    }
+  }
+  const int n_items = (int)items.size();
+  const int grid = std::max(1, std::min(n_items, max_blocks));
+  ctx->stats[2] = n_items;
+  ctx->stats[3] = grid;
+  ctx->stats[7] = block;
+
+  // ---- upload
+  HIPCHK(ctx->slices.ensure(n));
+  HIPCHK(ctx->items.ensure(items.size()));
+  HIPCHK(ctx->cells.ensure(cells.size()));
+  HIPCHK(ctx->dedges.ensure(dedges.size()));
+  HIPCHK(ctx->vedges.ensure(vedges.size()));
+  HIPCHK(ctx->vedges32.ensure(vedges32.size()));
+  HIPCHK(ctx->acc_s.ensure((size_t)n * ncell));
+  HIPCHK(ctx->acc_c.ensure((size_t)n * ncell));
+  HIPCHK(ctx->counter.ensure(4));
+  HIPCHK(ctx->pcount.ensure(4));
+  HIPCHK(hipMemcpyAsync(ctx->slices.p, sdesc.data(), sizeof(Slice) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->items.p, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->cells.p, cells.data(), sizeof(Cell) * cells.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->dedges.p, dedges.data(), sizeof(double) * dedges.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->vedges.p, vedges.data(), sizeof(double) * vedges.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->vedges32.p, vedges32.data(), sizeof(float) * vedges32.size(), hipMemcpyHostToDevice, st));
+
+  int64_t cap = 0;
+  if (mixed) {
+    cap = std::max<int64_t>(1LL << 22, total_pairs / 2000);
+    cap = std::min<int64_t>(cap, 1LL << 27);
+    HIPCHK(ctx->pending.ensure((size_t)cap));
+  }
+
+  Params P{};
+  P.slices = ctx->slices.p;
+  P.items = ctx->items.p;
+  P.n_items = n_items;
+  P.block_rows = block;
+  P.counter = ctx->counter.p;
+  P.rec64 = ctx->rec64.p;
+  P.rec32 = ctx->rec32.p;
+  P.cells = ctx->cells.p;
+  P.dedges = ctx->dedges.p;
+  P.vedges = ctx->vedges.p;
+  P.vedges32 = ctx->vedges32.p;
+  P.sums = ctx->acc_s.p;
+  P.counts = ctx->acc_c.p;
+  P.pending = ctx->pending.p;
+  P.pending_count = ctx->pcount.p;
+  P.pending_cap = cap;
+  P.stats = nullptr;
+  P.ncell_max = ncell_max;
+  P.n_dist = p0.n_dist;
+  P.n_val = p0.n_val;
+  P.op = p0.op;
+  P.nl = p0.nl;
+  P.nt = p0.nt;
+  P.order = op_order(p0);
+  P.radius = (p0.geometry == SF_GEOM_SPHERE_DEG) ? p0.radius : 1.0;
+  // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + 4 / sin(sigma))
+  P.vcoef = (float)ldexp((flat ? 64.0 : 128.0) * std::max(1, P.order), -24);
+
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool use_mixed = mixed && attempt == 0;
+    HIPCHK(launch_zero(ctx->acc_s.p, sizeof(double) * (int64_t)n * ncell, st));
+    HIPCHK(launch_zero(ctx->acc_c.p, sizeof(unsigned long long) * (int64_t)n * ncell, st));
+    HIPCHK(launch_zero(ctx->counter.p, sizeof(int) * 4, st));
+    HIPCHK(launch_zero(ctx->pcount.p, sizeof(unsigned long long) * 4, st));
+    const int stage = use_mixed ? block * F32 * 4 : block * S64 * 8;
+    const SmemLayout L2 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage);
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(launch_hist(G, D, J, use_mixed, P, grid, block, (size_t)L2.total, st));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    if (use_mixed) HIPCHK(launch_recheck(G, D, J, P, 4 * ctx->n_cu, st));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    if (!use_mixed) break;
+    unsigned long long pc = 0;
+    HIPCHK(hipMemcpyAsync(&pc, ctx->pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats[1] = (int64_t)pc;
+    if ((int64_t)pc <= cap) break;
+    // queue overflow: discard and redo with the binary64 kernel
+  }
+  HIPCHK(launch_merge(ctx->acc_s.p, ctx->acc_c.p, out_s, out_c, (int64_t)n * ncell, st));
+  HIPCHK(hipEventRecord(ctx->ev[3], st));
+  ctx->ev_pending = true;
+  return SF_OK;
+}
+
+int host_entry(sf_ctx *ctx, int n, const sf_problem *sl, double *sums, uint64_t *counts) {
+  ctx->err.clear();
+  int rc = validate_batch(ctx, n, sl, true);
+  if (rc) return rc;
+  if (n == 0) return SF_OK;
+  if (!sums || !counts) return ctx->fail(SF_E_ARG, "null sums or counts");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const sf_problem &p0 = sl[0];
+  const int ncell = p0.n_dist * (p0.n_val > 0 ? p0.n_val : 1);
+  int64_t tx = 0, tu = 0;
+  for (int s = 0; s < n; ++s) {
+    tx += (int64_t)sl[s].W * sl[s].N;
+    tu += (int64_t)sl[s].D * sl[s].N;
+  }
+  HIPCHK(ctx->d_x.ensure((size_t)tx + 2));
+  HIPCHK(ctx->d_u.ensure((size_t)tu + 2));
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  int64_t ox = 0, ou = 0;
+  for (int s = 0; s < n; ++s) {
+    const int64_t nx = (int64_t)sl[s].W * sl[s].N, nu = (int64_t)sl[s].D * sl[s].N;
+    if (nx) HIPCHK(hipMemcpyAsync(ctx->d_x.p + ox, sl[s].x, sizeof(double) * nx, hipMemcpyHostToDevice, st));
+    if (nu) HIPCHK(hipMemcpyAsync(ctx->d_u.p + ou, sl[s].u, sizeof(double) * nu, hipMemcpyHostToDevice, st));
+    px[s] = ctx->d_x.p + ox;
+    pu[s] = ctx->d_u.p + ou;
+    double m = 0;
+    for (int64_t k = 0; k < nx; ++k) m = std::max(m, fabs(sl[s].x[k]));
+    xmax[s] = m;
+    ox += nx;
+    ou += nu;
+  }
+  // outputs: accumulate into zeroed device staging, then add on the host
+  DevBuf<double> os;
+  DevBuf<unsigned long long> oc;
+  HIPCHK(os.ensure((size_t)n * ncell));
+  HIPCHK(oc.ensure((size_t)n * ncell));
+  HIPCHK(launch_zero(os.p, sizeof(double) * (int64_t)n * ncell, st));
+  HIPCHK(launch_zero(oc.p, sizeof(unsigned long long) * (int64_t)n * ncell, st));
+  rc = run_batch(ctx, n, sl, px.data(), pu.data(), xmax.data(), os.p, oc.p, st);
+  if (rc) { os.release(); oc.release(); return rc; }
+  std::vector<double> hs((size_t)n * ncell);
+  std::vector<unsigned long long> hc((size_t)n * ncell);
+  hipError_t e1 = hipMemcpyAsync(hs.data(), os.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st);
+  hipError_t e2 = hipMemcpyAsync(hc.data(), oc.p, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st);
+  hipError_t e3 = hipStreamSynchronize(st);
+  os.release();
+  oc.release();
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    return ctx->fail(SF_E_DEVICE, "device to host copy failed: %s",
+                     hipGetErrorString(e3 != hipSuccess ? e3 : (e1 != hipSuccess ? e1 : e2)));
+  for (size_t k = 0; k < hs.size(); ++k) {
+    sums[k] += hs[k];
+    counts[k] += hc[k];
+  }
+  return SF_OK;
+}
+
+}  // namespace
+
+extern "C" int sf_hist(sf_ctx *ctx, const sf_problem *p, double *sums, uint64_t *counts) {
+  if (!ctx) return SF_E_ARG;
+  return host_entry(ctx, 1, p, sums, counts);
+}
+
+extern "C" int sf_hist_batch(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *sums,
+                             uint64_t *counts) {
+  if (!ctx) return SF_E_ARG;
+  return host_entry(ctx, n_slices, slices, sums, counts);
+}
+
+extern "C" int sf_hist_batch_device(sf_ctx *ctx, int32_t n, const sf_problem *sl, double *d_sums,
+                                    uint64_t *d_counts, void *stream) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  int rc = validate_batch(ctx, n, sl, false);
+  if (rc) return rc;
+  if (n == 0) return SF_OK;
+  if (!d_sums || !d_counts) return ctx->fail(SF_E_ARG, "null sums or counts");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+    // coordinate magnitude for the binary32 margin: read x back (O(N)) only for the mixed flat path
+    if (sl[s].precision == SF_PREC_MIXED && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+      std::vector<double> hx((size_t)sl[s].W * sl[s].N);
+      HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double m = 0;
+      for (double v : hx) m = std::max(m, fabs(v));
+      xmax[s] = m;
+    }
+  }
+  return run_batch(ctx, n, sl, px.data(), pu.data(), xmax.data(), d_sums,
+                   (unsigned long long *)d_counts, st);
+}
+
+extern "C" int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d_counts,
+                              void *stream) {
+  return sf_hist_batch_device(ctx, 1, p, d_sums, d_counts, stream);
+}

@@ -1,0 +1,119 @@
+// sf_internal.h -- layouts shared by the host launcher (sf_api.cpp) and the kernels (sf_kernels.hip).
+#pragma once
+#include <stdint.h>
+
+#if !defined(__HIPCC__)
+#define __host__
+#define __device__
+#endif
+
+namespace sfk {
+
+enum Geom : int { kFlat = 0, kSphere = 1 };
+
+// Per-point records (array of structures, 16-byte aligned) built on the device by the prep kernels.
+//   binary64 record (exact path):
+//     flat   D=2: x0 x1 u0 u1                      (4 doubles)
+//     flat   D=3: x0 x1 x2 u0 u1 u2                (6 doubles)
+//     sphere D  : P0 P1 P2 E0 E1 N0 N1 N2 u0 u1 [u2] pad -> 12 doubles; P = unit position,
+//                 E = local east (E2 = 0), N = local north (HelperFunctions.jl:133-136)
+//   binary32 record (mixed path), every binary64 input carried as a hi + lo pair of floats
+//   (v = hi + lo to ~2^-48 relative) so that differences of nearby values stay accurate:
+//     flat   D=2: xh0 xh1 xl0 xl1 uh0 uh1 ul0 ul1 |u| 0 0 0                   (12 floats)
+//     flat   D=3: xh0..2 xl0..2 uh0..2 ul0..2 |u| 0 0 0                       (16 floats)
+//     sphere D  : ph0..2 pl0..2 Uh0..2 Ul0..2 |u| urh url 0                  (16 floats)
+//                 U = u_e E + u_n N is the tangent velocity in R^3, ur the radial component
+//                 (D = 3), P the unit position; |u| is an upper bound of the velocity norm.
+template <int G, int D> struct Rec {
+  static constexpr int d64 = (G == kFlat) ? ((D == 2) ? 4 : 6) : 12;
+  static constexpr int f32 = (G == kFlat && D == 2) ? 12 : 16;
+};
+
+// Distance-key cell table entry.  The positive key axis (flat: r^2; sphere: chord^2 for the
+// binary32 path, central-angle distance for the binary64 path) is cut into cells at powers of two
+// times (1 + m / 2^M); a key's cell is (bits >> (mantissa_bits - M)) - base, clamped.
+struct Cell {
+  float lo, hi;   // binary32 ambiguity interval inside this cell (lo > hi: none; -inf..inf: all)
+  int32_t blo;    // binary32 path: bin count for keys below lo (keys above hi: blo + 1)
+  int32_t bx;     // binary64 path: number of exact edges below the cell's lower bound
+};
+
+struct Slice {
+  int64_t N;          // points
+  int64_t pt_off;     // first record of this slice in the record arrays
+  int64_t out_off;    // first output cell
+  int64_t base64;     // binary64 table: cell = (bits >> (52 - M64)) - base64
+  int32_t M64, ncell64;
+  int32_t M32, ncell32;
+  int32_t base32;     // binary32 table: cell = (bits >> (23 - M32)) - base32
+  int32_t tab_off;    // first Cell of this slice's table (binary64 keys)
+  int32_t tab32_off;  // first Cell of this slice's binary32 (chord^2 / r^2) table
+  int32_t dedge_off;  // exact distance-key edges (flat: e*e; sphere: e), n_dist + 1 doubles
+  int32_t vedge_off;  // value edges, n_val + 1 doubles (and float copies at the same offset)
+  int32_t v_uniform;  // value edges uniform enough for the guess-then-walk lookup
+  int32_t v_negzero;  // count of -0.0 value edges (Julia isless places +0.0 above them)
+  double v0, vinv;    // value guess t = (v - v0) * vinv
+  float v0f, vinvf;   // binary32 value guess
+  float vslack;       // binary32 value-axis slack in units of t (edge non-uniformity + rounding)
+  float pad0;
+};
+
+struct Item {
+  int32_t slice;
+  int32_t i0;       // first row of the i-tile
+  int32_t j0, j1;   // column range [j0, j1), j0 >= i0
+};
+
+struct Pending {      // a pair the binary32 kernel could not bin with certainty
+  int32_t slice, i, j, pad;
+};
+
+struct Params {
+  const Slice *slices;
+  const Item *items;
+  int32_t n_items;
+  int32_t block_rows;      // i-tile rows (= threads per block)
+  int32_t *counter;        // work-queue head (zeroed before launch)
+  const double *rec64;     // binary64 records
+  const float *rec32;      // binary32 records
+  const Cell *cells;       // cell tables (binary64 tables and binary32 tables)
+  const double *dedges;    // exact distance-key edges
+  const double *vedges;    // value edges (binary64)
+  const float *vedges32;   // value edges rounded to binary32 (non-uniform value lookup)
+  double *sums;            // output sums (device)
+  unsigned long long *counts;  // output counts (device)
+  Pending *pending;        // re-bin queue (mixed kernel)
+  unsigned long long *pending_count;  // [0] = appended, may exceed capacity
+  int64_t pending_cap;
+  unsigned long long *stats;  // [0] pairs visited by exact kernel recheck
+  int32_t ncell_max;       // largest table of any slice (LDS sizing)
+  int32_t n_dist, n_val;
+  int32_t op, nl, nt;
+  int32_t order;           // operator order (value-error bound exponent)
+  double radius;
+  float vcoef;             // binary32 value error coefficient: |dv| <= vcoef * Umax^order * amp
+  float pad1;
+};
+
+// LDS layout of the histogram kernels, shared by the host (sizing) and the device (carving).
+struct SmemLayout {
+  int32_t hs, hc, tab, dE, vE, vE32, stage, total;  // byte offsets
+};
+
+inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
+                                                  int stage_bytes) {
+  SmemLayout L;
+  const int nh = n_dist * (n_val > 0 ? n_val : 1);
+  int o = 0;
+  L.hs = o; o += 8 * nh;
+  L.hc = o; o += 4 * ((nh + 3) & ~3);
+  L.tab = o; o += 16 * ncell_max;
+  L.dE = o; o += 8 * ((n_dist + 2) & ~1);
+  L.vE = o; o += 8 * ((n_val + 2) & ~1);
+  L.vE32 = o; o += 4 * ((n_val + 4) & ~3);
+  L.stage = o; o += stage_bytes;
+  L.total = o;
+  return L;
+}
+
+}  // namespace sfk

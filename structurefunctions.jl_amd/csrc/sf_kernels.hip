@@ -1,0 +1,916 @@
+// sf_kernels.hip -- gfx950 kernels for the O(N^2) pair histogram.
+//
+// Kernels
+//   prep64<G,D>      per-point binary64 records (sphere: sincosd once per point, not per pair)
+//   prep32<G,D>      per-point binary32 records for the mixed kernel
+//   hist64<G,D,J>    persistent upper-triangle tile kernel, binary64 pair arithmetic written in
+//                    the reference's operation order: counts bit-identical to SerialBackend
+//   hist32<G,D,J>    same schedule, binary32 pair arithmetic; a pair whose distance key or value
+//                    lies within the binary32 error bound of a bin edge (or is degenerate) is
+//                    appended to a queue instead of being binned
+//   recheck<G,D,J>   bins every queued pair with the binary64 arithmetic of hist64
+//   merge            adds the internal accumulators into the caller's outputs
+//
+// Schedule (both hist kernels): a work item is (slice, i-tile of BLOCK rows, column range
+// [j0, j1)).  Each thread owns one row i and keeps its record in registers; the block stages
+// BLOCK column records at a time in LDS and every wave reads them as broadcasts.  Pairs are
+// (i, j) with j > i.  Workgroups are persistent and pop items from a global counter; each keeps
+// the current slice's histogram in LDS (binary64 sums, 32-bit counts) and flushes it to the
+// accumulators with global atomics when it changes slice, before its 32-bit counts could wrap,
+// and when it runs out of items.
+//
+// All files of this library are compiled with -ffp-contract=off: the binary64 path must perform
+// exactly the operations of the reference (src/HelperFunctions.jl, src/StructureFunctionTypes.jl,
+// src/BinEdges.jl) with no fused multiply-adds.  The binary32 path asks for fma explicitly.
+#include <hip/hip_runtime.h>
+
+#include "sf_internal.h"
+#include "sfmath.h"
+
+namespace sfk {
+
+#define DEV __device__ __forceinline__
+
+// =============================================================================================
+// binary64 exact per-pair arithmetic (mirrors the oracle, which mirrors the reference)
+// =============================================================================================
+
+// StaticArrays dot: left fold from the first product
+template <int D>
+DEV double sdot(const double *a, const double *b) {
+  double s = a[0] * b[0];
+#pragma unroll
+  for (int k = 1; k < D; ++k) s = s + a[k] * b[k];
+  return s;
+}
+
+// _sum_abs2 / norm2: 0 + a1^2 + ...
+template <int D>
+DEV double sum_abs2(const double *a) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) s += a[k] * a[k];
+  return s;
+}
+
+DEV double ipow(double x, int n) {
+  if (n == 1) return x;
+  if (n == 2) return x * x;
+  return x * x * x;
+}
+
+// mdu_t = dot(du, nhat(rhat))   (HelperFunctions.jl:450-471, 640-643)
+template <int D>
+DEV double mdu_t64(const double *du, const double *rh) {
+  if (D == 2) {
+    const double n[2] = {-rh[1], rh[0]};
+    return sdot<2>(du, n);
+  } else {
+    const double c[3] = {0.0 * rh[2] - 1.0 * rh[1], 1.0 * rh[0] - 0.0 * rh[2],
+                         0.0 * rh[1] - 0.0 * rh[0]};
+    const double nrm = __builtin_sqrt(sdot<3>(c, c));
+    const double n[3] = {c[0] / nrm, c[1] / nrm, c[2] / nrm};
+    return sdot<3>(du, n);
+  }
+}
+
+// operator functor sf(du, rhat)   (StructureFunctionTypes.jl:58-151)
+template <int D>
+DEV double op_apply64(int op, int nl, int nt, const double *du, const double *rh) {
+  switch (op) {
+    case 0: {  // ProjectedStructureFunctionType{nl, nt}
+      double ex = 1.0;
+      if (nl != 0) ex = ex * ipow(sdot<D>(du, rh), nl);
+      if (nt != 0) {
+        if (nt == 2) {
+          const double L = sdot<D>(du, rh);
+          ex = ex * (sum_abs2<D>(du) - L * L);
+        } else {
+          ex = ex * ipow(mdu_t64<D>(du, rh), nt);
+        }
+      }
+      return ex;
+    }
+    case 1:  // SecondOrder
+      return sum_abs2<D>(du);
+    case 2:  // ThirdOrder: du_L * ||du||^2
+      return sdot<D>(du, rh) * sum_abs2<D>(du);
+    case 3: {  // T2Component
+      const double L = sdot<D>(du, rh);
+      return (sum_abs2<D>(du) - L * L) / (double)(D - 1);
+    }
+    case 4: {  // L1T2Component
+      const double L = sdot<D>(du, rh);
+      return L * ((sum_abs2<D>(du) - L * L) / (double)(D - 1));
+    }
+    default: {  // FullVector{nl}
+      if (nl == 2) return sum_abs2<D>(du);
+      return ipow(__builtin_sqrt(sum_abs2<D>(du)), nl);
+    }
+  }
+}
+
+// _sf_raw(sf, du, dx, r2) on the flat path   (StructureFunctionTypes.jl:227-244)
+template <int D>
+DEV double sf_raw64(int op, int nl, int nt, const double *du, const double *dx, double r2) {
+  if (op == 1) return sum_abs2<D>(du);
+  if (op == 0 && nl == 2 && nt == 0) {
+    const double p = sdot<D>(du, dx);
+    return p * p / r2;
+  }
+  if (op == 0 && nl == 0 && nt == 2) {
+    const double p = sdot<D>(du, dx);
+    return sum_abs2<D>(du) - p * p / r2;
+  }
+  if (op == 3) {
+    const double p = sdot<D>(du, dx);
+    return (sum_abs2<D>(du) - p * p / r2) / (double)(D - 1);
+  }
+  const double s = __builtin_sqrt(r2);
+  double rh[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) rh[k] = dx[k] / s;
+  return op_apply64<D>(op, nl, nt, du, rh);
+}
+
+// Spherical pair frame (HelperFunctions.jl:186-200, 265-276) from binary64 records
+// (P0 P1 P2 E0 E1 N0 N1 N2 u...): distance, ok, and the pieces pair_delta needs.
+struct SphereFrame {
+  double d[3];
+  double dp, s2, distance;
+  bool ok;
+};
+
+DEV SphereFrame sphere_frame64(const double *p, const double *q, double radius) {
+  SphereFrame f;
+  f.d[0] = q[0] - p[0]; f.d[1] = q[1] - p[1]; f.d[2] = q[2] - p[2];
+  const double s[3] = {p[0] + q[0], p[1] + q[1], p[2] + q[2]};
+  f.dp = sdot<3>(f.d, p);
+  const double *d = f.d;
+  const double w[3] = {p[1] * d[2] - p[2] * d[1], p[2] * d[0] - p[0] * d[2],
+                       p[0] * d[1] - p[1] * d[0]};
+  f.s2 = sdot<3>(w, w);
+  const double sigma = 2.0 * sf_atan2(__builtin_sqrt(sdot<3>(d, d)), __builtin_sqrt(sdot<3>(s, s)));
+  f.ok = f.s2 > 2.2250738585072014e-308;  // floatmin(Float64)
+  f.distance = radius * sigma;
+  return f;
+}
+
+// pair_delta(::SphericalGeometry{D}) (HelperFunctions.jl:324-348) given the frame
+template <int D>
+DEV void sphere_delta64(const SphereFrame &f, const double *A, const double *B, double *du) {
+  const double *p = A, *q = B, *d = f.d;
+  const double dp = f.dp;
+  const double t_A[3] = {d[0] - dp * p[0], d[1] - dp * p[1], d[2] - dp * p[2]};
+  const double t_B[3] = {dp * q[0] + d[0], dp * q[1] + d[1], dp * q[2] + d[2]};
+  const double inv_s = f.ok ? 1.0 / __builtin_sqrt(f.s2) : 0.0;
+  const double ta[3] = {t_A[0] * inv_s, t_A[1] * inv_s, t_A[2] * inv_s};
+  const double tb[3] = {t_B[0] * inv_s, t_B[1] * inv_s, t_B[2] * inv_s};
+  const double EA[3] = {A[3], A[4], 0.0}, NA[3] = {A[5], A[6], A[7]};
+  const double EB[3] = {B[3], B[4], 0.0}, NB[3] = {B[5], B[6], B[7]};
+  const double tA[2] = {sdot<3>(ta, EA), sdot<3>(ta, NA)};
+  const double tB[2] = {sdot<3>(tb, EB), sdot<3>(tb, NB)};
+  const double nA[2] = {-tA[1], tA[0]};
+  const double nB[2] = {-tB[1], tB[0]};
+  const double *uA = A + 8, *uB = B + 8;
+  du[0] = sdot<2>(uB, tB) - sdot<2>(uA, tA);
+  du[1] = sdot<2>(uB, nB) - sdot<2>(uA, nA);
+  if (D == 3) du[2] = uB[2] - uA[2];
+}
+
+// ---------------------------------------------------------------------------------------------
+// exact bin lookup
+// ---------------------------------------------------------------------------------------------
+DEV int cell64(double key, const Slice &s) {
+  const long long bits = __double_as_longlong(key);
+  long long c = (bits >> (52 - s.M64)) - s.base64;
+  c = c < 0 ? 0 : c;
+  c = c >= s.ncell64 ? s.ncell64 - 1 : c;
+  return (int)c;
+}
+
+// number of exact distance-key edges strictly below key (key > 0), 0 for key <= 0, nE for NaN.
+DEV int dist_bin64(double key, const Cell *tab, const double *E, int nE, const Slice &s) {
+  if (!(key > 0.0)) return (key != key) ? nE : 0;
+  int b = tab[cell64(key, s)].bx;
+  while (b < nE && E[b] < key) ++b;
+  return b;
+}
+
+// searchsortedfirst(value_edges, v) - 1 with Julia isless semantics
+DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
+  if (v != v) return nE;
+  int g;
+  if (s.v_uniform) {
+    double t = (v - s.v0) * s.vinv;
+    t = t < -1.0 ? -1.0 : t;
+    t = t > (double)nE ? (double)nE : t;
+    g = (int)__builtin_floor(t) + 1;
+    g = g < 0 ? 0 : (g > nE ? nE : g);
+    while (g > 0 && !(E[g - 1] < v)) --g;
+    while (g < nE && E[g] < v) ++g;
+  } else {
+    int lo = 0, hi = nE;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (E[m] < v) lo = m + 1; else hi = m;
+    }
+    g = lo;
+  }
+  if (v == 0.0 && !sf_signbit(v)) g += s.v_negzero;
+  return g;
+}
+
+// Full exact evaluation of pair (i, j) of slice s from binary64 records.  Returns the output cell
+// (0-based within the slice) or -1 if the pair is not counted; writes the value.
+template <int G, int D, bool J>
+DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
+                   const Cell *tab, const double *dE, const double *vE, double *val) {
+  const int nEd = P.n_dist + 1;
+  int db;
+  double v;
+  if (G == kFlat) {
+    double dx[D], du[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      dx[k] = B[k] - A[k];
+      du[k] = B[D + k] - A[D + k];
+    }
+    const double r2 = sdot<D>(dx, dx);
+    db = dist_bin64(r2, tab, dE, nEd, s);
+    if (db < 1 || db > P.n_dist) return -1;
+    v = sf_raw64<D>(P.op, P.nl, P.nt, du, dx, r2);
+  } else {
+    const SphereFrame f = sphere_frame64(A, B, P.radius);
+    db = dist_bin64(f.distance, tab, dE, nEd, s);
+    if (!f.ok || db < 1 || db > P.n_dist) return -1;
+    double du[3];
+    sphere_delta64<D>(f, A, B, du);
+    const double rh[3] = {1.0, 0.0, 0.0};
+    v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+  }
+  *val = v;
+  if (!J) return db - 1;
+  const int vb = val_bin64(v, vE, P.n_val + 1, s);
+  if (vb < 1 || vb > P.n_val) return -1;
+  return (db - 1) + P.n_dist * (vb - 1);
+}
+
+// =============================================================================================
+// prep kernels
+// =============================================================================================
+struct PrepSlice {
+  const double *x, *u;
+  int64_t N, pt_off;
+  double xscale;  // unused placeholder (kept for alignment)
+};
+
+template <int G, int D>
+__global__ void prep64(const PrepSlice *ps, int geom_code, double *rec64) {
+  const PrepSlice s = ps[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.N) return;
+  constexpr int S = Rec<G, D>::d64;
+  double *r = rec64 + (s.pt_off + i) * S;
+  if (G == kFlat) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      r[k] = s.x[D * i + k];
+      r[D + k] = s.u[D * i + k];
+    }
+  } else {
+    const double lon = s.x[2 * i], lat = s.x[2 * i + 1];
+    double sl, cl, sp, cp;
+    if (geom_code == 1) {
+      sf_sincosd(lon, &sl, &cl);
+      sf_sincosd(lat, &sp, &cp);
+    } else {
+      sf_sincos(lon, &sl, &cl);
+      sf_sincos(lat, &sp, &cp);
+    }
+    r[0] = cp * cl; r[1] = cp * sl; r[2] = sp;     // unit position
+    r[3] = -sl; r[4] = cl;                          // east (third component 0)
+    r[5] = -sp * cl; r[6] = -sp * sl; r[7] = cp;   // north
+    r[8] = s.u[D * i]; r[9] = s.u[D * i + 1];
+    r[10] = (D == 3) ? s.u[D * i + 2] : 0.0;
+    r[11] = 0.0;
+  }
+}
+
+// binary32 value >= x for x >= 0 (norms used in error bounds)
+DEV float round_up_f(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = __int_as_float(__float_as_int(f) + 1);
+  return f;
+}
+
+DEV void split2(double v, float *hi, float *lo) {
+  const float h = (float)v;
+  *hi = h;
+  *lo = (float)(v - (double)h);
+}
+
+template <int G, int D>
+__global__ void prep32(const PrepSlice *ps, const double *rec64, float *rec32) {
+  const PrepSlice s = ps[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= s.N) return;
+  constexpr int S = Rec<G, D>::d64;
+  constexpr int F = Rec<G, D>::f32;
+  const double *r = rec64 + (s.pt_off + i) * S;
+  float *o = rec32 + (s.pt_off + i) * F;
+  for (int k = 0; k < F; ++k) o[k] = 0.0f;
+  if (G == kFlat) {
+    double n2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      split2(r[k], &o[k], &o[D + k]);
+      split2(r[D + k], &o[2 * D + k], &o[3 * D + k]);
+      n2 += r[D + k] * r[D + k];
+    }
+    o[4 * D] = round_up_f(__builtin_sqrt(n2) * (1.0 + 1e-12));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) split2(r[k], &o[k], &o[3 + k]);
+    const double ue = r[8], un = r[9];
+    const double E[3] = {r[3], r[4], 0.0}, Nn[3] = {r[5], r[6], r[7]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) split2(ue * E[k] + un * Nn[k], &o[6 + k], &o[9 + k]);
+    const double n2 = ue * ue + un * un + (D == 3 ? r[10] * r[10] : 0.0);
+    o[12] = round_up_f(__builtin_sqrt(n2) * (1.0 + 1e-12));
+    split2(r[10], &o[13], &o[14]);
+  }
+}
+
+// =============================================================================================
+// persistent histogram kernels
+// =============================================================================================
+struct Smem {
+  double *hs;
+  unsigned *hc;
+  Cell *tab;
+  double *dE;
+  double *vE;
+  float *vE32;
+  char *stage;
+  int ncell_h;
+};
+
+DEV Smem carve(char *base, const Params &P, int stage_bytes) {
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes);
+  Smem m;
+  m.ncell_h = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
+  m.hs = (double *)(base + L.hs);
+  m.hc = (unsigned *)(base + L.hc);
+  m.tab = (Cell *)(base + L.tab);
+  m.dE = (double *)(base + L.dE);
+  m.vE = (double *)(base + L.vE);
+  m.vE32 = (float *)(base + L.vE32);
+  m.stage = base + L.stage;
+  return m;
+}
+
+DEV void zero_hist(const Smem &m) {
+  for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
+    m.hs[c] = 0.0;
+    m.hc[c] = 0u;
+  }
+}
+
+DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
+  for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
+    const unsigned n = m.hc[c];
+    if (n) {
+      atomicAdd(&P.counts[s.out_off + c], (unsigned long long)n);
+      atomicAdd(&P.sums[s.out_off + c], m.hs[c]);
+      m.hs[c] = 0.0;
+      m.hc[c] = 0u;
+    }
+  }
+}
+
+DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32) {
+  const Cell *src = P.cells + (fast32 ? s.tab32_off : s.tab_off);
+  const int nc = fast32 ? s.ncell32 : s.ncell64;
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) m.tab[c] = src[c];
+  for (int k = threadIdx.x; k <= P.n_dist; k += blockDim.x) m.dE[k] = P.dedges[s.dedge_off + k];
+  if (P.n_val > 0) {
+    for (int k = threadIdx.x; k <= P.n_val; k += blockDim.x) {
+      m.vE[k] = P.vedges[s.vedge_off + k];
+      m.vE32[k] = P.vedges32[s.vedge_off + k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// hist64: binary64 exact pair arithmetic for every pair
+// ---------------------------------------------------------------------------------------------
+template <int G, int D, bool J, bool MASK>
+DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i,
+                     int jb, int cnt) {
+  constexpr int S = Rec<G, D>::d64;
+  const double *st = (const double *)m.stage;
+  for (int jj = 0; jj < cnt; ++jj) {
+    const int j = jb + jj;
+    if (MASK && j <= i) continue;
+    double rj[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) rj[k] = st[jj * S + k];
+    double v;
+    const int c = exact_cell<G, D, J>(P, s, ri, rj, m.tab, m.dE, m.vE, &v);
+    if (c >= 0) {
+      atomicAdd(&m.hs[c], v);
+      atomicAdd(&m.hc[c], 1u);
+    }
+  }
+}
+
+template <int G, int D, bool J>
+__global__ void __launch_bounds__(512) hist64(Params P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  constexpr int S = Rec<G, D>::d64;
+  const Smem m = carve(smem_raw, P, 0);
+  __shared__ int s_item;
+  const int B = blockDim.x;
+  int cur = -1;
+  Slice s{};
+  unsigned long long since_flush = 0;
+  zero_hist(m);
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= P.n_items) break;
+    const Item w = P.items[it];
+    const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
+    if (w.slice != cur || since_flush + work > 0x7fffffffull) {
+      if (cur >= 0) {
+        flush_hist(m, P, s);
+        __syncthreads();
+      }
+      if (w.slice != cur) {
+        cur = w.slice;
+        s = P.slices[cur];
+        load_tables(m, P, s, false);
+      }
+      since_flush = 0;
+    }
+    since_flush += work;
+    const int i = w.i0 + (int)threadIdx.x;
+    const bool iv = i < s.N;
+    double ri[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) ri[k] = iv ? P.rec64[(s.pt_off + i) * S + k] : 0.0;
+    for (int jb = w.j0; jb < w.j1; jb += B) {
+      const int cnt = min(B, w.j1 - jb);
+      __syncthreads();
+      const double *src = P.rec64 + (s.pt_off + jb) * S;
+      double *dst = (double *)m.stage;
+      for (int k = threadIdx.x; k < cnt * S; k += B) dst[k] = src[k];
+      __syncthreads();
+      if (iv) {
+        if (jb < w.i0 + B) run_chunk64<G, D, J, true>(P, s, m, ri, i, jb, cnt);
+        else run_chunk64<G, D, J, false>(P, s, m, ri, i, jb, cnt);
+      }
+    }
+  }
+  __syncthreads();
+  if (cur >= 0) flush_hist(m, P, s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// hist32: binary32 pair arithmetic with certified binning, uncertain pairs queued
+// ---------------------------------------------------------------------------------------------
+DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+DEV int cell32(float key, const Slice &s) {
+  const int bits = __float_as_int(key);
+  int c = (bits >> (23 - s.M32)) - s.base32;
+  c = c < 0 ? 0 : c;
+  c = c >= s.ncell32 ? s.ncell32 - 1 : c;
+  return c;
+}
+
+// value-axis bin from a binary32 value with absolute error bound dv; sets amb when undecidable.
+DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, bool &amb) {
+  if (!(__builtin_fabsf(v) < __builtin_inff())) {
+    amb = true;
+    return 0;
+  }
+  if (s.v_uniform) {
+    const float t = (v - s.v0f) * s.vinvf;
+    const float tc = __builtin_fminf(__builtin_fmaxf(t, -2.0f), (float)nE + 1.0f);
+    const float g = __builtin_floorf(tc);
+    const float fr = tc - g;
+    // error of t: value bound, host slack (edge non-uniformity, rounding of v0 and 1/w), and
+    // the rounding of the subtraction and product (a few ulp of |t|)
+    const float eps = dv * __builtin_fabsf(s.vinvf) + s.vslack + 4.8e-7f * __builtin_fabsf(tc);
+    amb = amb || (fr <= eps) || (fr >= 1.0f - eps);
+    return (int)g + 1;
+  }
+  int lo = 0, hi = nE;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (E32[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  // nearest edges must be farther than the error bound (edges rounded to binary32: slack)
+  const float gap = dv + s.vslack * __builtin_fabsf(v) + 1e-37f;
+  if (lo > 0 && v - E32[lo - 1] <= gap) amb = true;
+  if (lo < nE && E32[lo] - v <= gap) amb = true;
+  return lo;
+}
+
+// binary32 value of the operator from du_L, du_T (signed transverse), ||du||^2 - du_L^2.
+DEV float op_value32(int op, int nl, int nt, int D, float L, float T, float tn2, float n2) {
+  switch (op) {
+    case 0: {
+      float ex = 1.0f;
+      if (nl == 1) ex = L;
+      else if (nl == 2) ex = L * L;
+      else if (nl == 3) ex = L * L * L;
+      if (nt == 1) ex = ex * T;
+      else if (nt == 2) ex = ex * tn2;
+      else if (nt == 3) ex = ex * (T * T * T);
+      return ex;
+    }
+    case 1: return n2;
+    case 2: return L * n2;
+    case 3: return tn2 / (float)(D - 1);
+    case 4: return L * (tn2 / (float)(D - 1));
+    default: {
+      if (nl == 2) return n2;
+      const float r = __builtin_sqrtf(n2);
+      return nl == 1 ? r : r * r * r;
+    }
+  }
+}
+
+template <int G, int D, bool J, bool MASK>
+DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
+                     int jb, int cnt, int slice_id) {
+  constexpr int F = Rec<G, D>::f32;
+  const float *st = (const float *)m.stage;
+  const int nb = P.n_dist;
+  for (int jj = 0; jj < cnt; ++jj) {
+    const int j = jb + jj;
+    if (MASK && j <= i) continue;
+    float rj[F];
+#pragma unroll
+    for (int k = 0; k < F; ++k) rj[k] = st[jj * F + k];
+    bool amb = false;
+    float key, L = 0.f, T = 0.f, tn2 = 0.f, n2 = 0.f, amp = 1.0f, umax;
+    if (G == kFlat) {
+      float dx[D], du[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
+        du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
+      }
+      key = dx[0] * dx[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) key = __builtin_fmaf(dx[k], dx[k], key);
+      umax = ri[4 * D] + rj[4 * D];
+      // distance bin
+      const Cell c = m.tab[cell32(key, s)];
+      amb = (key >= c.lo) && (key <= c.hi);
+      const int db = (key > c.hi) ? c.blo + 1 : c.blo;
+      if (!amb && (db < 1 || db > nb)) continue;
+      float p = du[0] * dx[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
+      n2 = du[0] * du[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
+      const float rinv = frsq(key);
+      L = p * rinv;
+      // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
+      // formed from the cross product so that it does not cancel when du is nearly longitudinal
+      const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
+      if (D == 2) {
+        T = cz * rinv;
+        tn2 = T * T;
+      } else {
+        T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
+        const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
+        const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
+        tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
+      }
+      amb = amb || !(key > 1e-30f);
+      const float v = op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
+      int cell = db - 1;
+      if (J) {
+        float up = 1.0f;
+        for (int e = 0; e < P.order; ++e) up *= umax;
+        const float dv = P.vcoef * up * amp;
+        const int vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
+        if (!amb && (vb < 1 || vb > P.n_val)) continue;
+        cell += nb * (vb - 1);
+      } else if (!(__builtin_fabsf(v) < __builtin_inff())) {
+        amb = true;
+      }
+      const unsigned long long mask = __ballot(amb);
+      if (mask) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (amb) {
+          const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+          if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
+          continue;
+        }
+      }
+      atomicAdd(&m.hs[cell], (double)v);
+      atomicAdd(&m.hc[cell], 1u);
+    } else {
+      // sphere: records ph[3] pl[3] Uh[3] Ul[3] |u| urh url 0
+      float d[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) d[k] = (rj[k] - ri[k]) + (rj[3 + k] - ri[3 + k]);
+      key = __builtin_fmaf(d[2], d[2], __builtin_fmaf(d[1], d[1], d[0] * d[0]));
+      umax = ri[12] + rj[12];
+      const Cell c = m.tab[cell32(key, s)];
+      amb = (key >= c.lo) && (key <= c.hi);
+      const int db = (key > c.hi) ? c.blo + 1 : c.blo;
+      if (!amb && (db < 1 || db > nb)) continue;
+      // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
+      //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
+      //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
+      //   du_T = w.(U_B - U_A)/sin
+      const float dp = -0.5f * key;
+      float tB[3], w[3], dU[3], sp[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
+        dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
+        sp[k] = ri[k] + rj[k];
+      }
+      w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
+      w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
+      w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
+      const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
+      amb = amb || !(s2 > 1e-30f);
+      const float inv_s = frsq(s2);
+      const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
+      const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
+      L = __builtin_fmaf(dp, sU, tdU) * inv_s;
+      T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
+      tn2 = T * T;
+      if (D == 3) {
+        const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
+        tn2 = __builtin_fmaf(dw, dw, tn2);
+      }
+      n2 = __builtin_fmaf(L, L, tn2);
+      amp = __builtin_fmaf(4.0f, inv_s, 1.0f);
+      const float v = op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
+      int cell = db - 1;
+      if (J) {
+        float up = 1.0f;
+        for (int e = 0; e < P.order; ++e) up *= umax;
+        const float dv = P.vcoef * up * amp;
+        const int vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
+        if (!amb && (vb < 1 || vb > P.n_val)) continue;
+        cell += nb * (vb - 1);
+      } else if (!(__builtin_fabsf(v) < __builtin_inff())) {
+        amb = true;
+      }
+      const unsigned long long mask = __ballot(amb);
+      if (mask) {
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (amb) {
+          const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+          if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
+          continue;
+        }
+      }
+      atomicAdd(&m.hs[cell], (double)v);
+      atomicAdd(&m.hc[cell], 1u);
+    }
+  }
+}
+
+template <int G, int D, bool J>
+__global__ void __launch_bounds__(512) hist32(Params P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  constexpr int F = Rec<G, D>::f32;
+  const Smem m = carve(smem_raw, P, 0);
+  __shared__ int s_item;
+  const int B = blockDim.x;
+  int cur = -1;
+  Slice s{};
+  unsigned long long since_flush = 0;
+  zero_hist(m);
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= P.n_items) break;
+    const Item w = P.items[it];
+    const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
+    if (w.slice != cur || since_flush + work > 0x7fffffffull) {
+      if (cur >= 0) {
+        flush_hist(m, P, s);
+        __syncthreads();
+      }
+      if (w.slice != cur) {
+        cur = w.slice;
+        s = P.slices[cur];
+        load_tables(m, P, s, true);
+      }
+      since_flush = 0;
+    }
+    since_flush += work;
+    const int i = w.i0 + (int)threadIdx.x;
+    const bool iv = i < s.N;
+    float ri[F];
+#pragma unroll
+    for (int k = 0; k < F; ++k) ri[k] = iv ? P.rec32[(s.pt_off + i) * F + k] : 0.0f;
+    for (int jb = w.j0; jb < w.j1; jb += B) {
+      const int cnt = min(B, w.j1 - jb);
+      __syncthreads();
+      const float4 *src = (const float4 *)(P.rec32 + (s.pt_off + jb) * F);
+      float4 *dst = (float4 *)m.stage;
+      for (int k = threadIdx.x; k < cnt * (F / 4); k += B) dst[k] = src[k];
+      __syncthreads();
+      if (iv) {
+        if (jb < w.i0 + B) run_chunk32<G, D, J, true>(P, s, m, ri, i, jb, cnt, cur);
+        else run_chunk32<G, D, J, false>(P, s, m, ri, i, jb, cnt, cur);
+      }
+    }
+  }
+  __syncthreads();
+  if (cur >= 0) flush_hist(m, P, s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// recheck: binary64 binning of queued pairs (global-memory tables, global atomics)
+// ---------------------------------------------------------------------------------------------
+template <int G, int D, bool J>
+__global__ void recheck(Params P) {
+  constexpr int S = Rec<G, D>::d64;
+  const unsigned long long n0 = *P.pending_count;
+  const unsigned long long n = n0 < (unsigned long long)P.pending_cap ? n0 : (unsigned long long)P.pending_cap;
+  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (unsigned long long)gridDim.x * blockDim.x) {
+    const Pending q = P.pending[k];
+    const Slice s = P.slices[q.slice];
+    double A[S], Bv[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      A[t] = P.rec64[(s.pt_off + q.i) * S + t];
+      Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
+    }
+    double v;
+    const int c = exact_cell<G, D, J>(P, s, A, Bv, P.cells + s.tab_off, P.dedges + s.dedge_off,
+                                      P.vedges + s.vedge_off, &v);
+    if (c >= 0) {
+      atomicAdd(&P.counts[s.out_off + c], 1ull);
+      atomicAdd(&P.sums[s.out_off + c], v);
+    }
+  }
+}
+
+__global__ void merge_out(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                          unsigned long long *dst_c, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    dst_s[k] += src_s[k];
+    dst_c[k] += src_c[k];
+  }
+}
+
+__global__ void zero_u64(unsigned long long *p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0ull;
+}
+
+__global__ void zero_f64(double *p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
+
+__global__ void eval_math(int fn, int64_t n, const double *a, const double *b, double *out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  switch (fn) {
+    case 0: sf_sincosd(a[k], &out[2 * k], &out[2 * k + 1]); break;
+    case 1: sf_sincos(a[k], &out[2 * k], &out[2 * k + 1]); break;
+    case 2: out[k] = sf_atan2(a[k], b[k]); break;
+    case 3: out[k] = __builtin_sqrt(a[k]); break;
+    default: out[k] = a[k] / b[k]; break;
+  }
+}
+
+__global__ void zero_i32(int *p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
+}  // namespace sfk
+
+// =============================================================================================
+// launchers (called from sf_api.hip)
+// =============================================================================================
+namespace sfk {
+
+struct PrepSliceHost;
+
+template <int G, int D>
+static hipError_t prep_impl(bool mixed, int geom_code, const void *ps, int n_slices, int64_t max_n,
+                            double *rec64, float *rec32, hipStream_t st) {
+  if (max_n <= 0) return hipSuccess;
+  const int T = 256;
+  dim3 grid((unsigned)((max_n + T - 1) / T), (unsigned)n_slices);
+  prep64<G, D><<<grid, T, 0, st>>>((const PrepSlice *)ps, geom_code, rec64);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !mixed) return e;
+  prep32<G, D><<<grid, T, 0, st>>>((const PrepSlice *)ps, rec64, rec32);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *ps, int n_slices,
+                       int64_t max_n, double *rec64, float *rec32, hipStream_t st) {
+  if (G == kFlat) {
+    return D == 2 ? prep_impl<kFlat, 2>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st)
+                  : prep_impl<kFlat, 3>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st);
+  }
+  return D == 2 ? prep_impl<kSphere, 2>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st)
+                : prep_impl<kSphere, 3>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st);
+}
+
+template <int G, int D, bool J>
+static const void *hist_fn(bool mixed) {
+  return mixed ? (const void *)&hist32<G, D, J> : (const void *)&hist64<G, D, J>;
+}
+
+static const void *pick_hist(int G, int D, bool J, bool mixed) {
+  if (G == kFlat) {
+    if (D == 2) return J ? hist_fn<kFlat, 2, true>(mixed) : hist_fn<kFlat, 2, false>(mixed);
+    return J ? hist_fn<kFlat, 3, true>(mixed) : hist_fn<kFlat, 3, false>(mixed);
+  }
+  if (D == 2) return J ? hist_fn<kSphere, 2, true>(mixed) : hist_fn<kSphere, 2, false>(mixed);
+  return J ? hist_fn<kSphere, 3, true>(mixed) : hist_fn<kSphere, 3, false>(mixed);
+}
+
+int hist_occupancy(int G, int D, bool J, bool mixed, int block, size_t smem) {
+  const void *f = pick_hist(G, D, J, mixed);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, smem) != hipSuccess) return 1;
+  return nb;
+}
+
+hipError_t launch_hist(int G, int D, bool J, bool mixed, const Params &P, int grid, int block,
+                       size_t smem, hipStream_t st) {
+  const void *f = pick_hist(G, D, J, mixed);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  Params Pc = P;
+  void *args[] = {&Pc};
+  return hipLaunchKernel(f, dim3(grid), dim3(block), args, smem, st);
+}
+
+template <int G, int D, bool J>
+static const void *recheck_fn() { return (const void *)&recheck<G, D, J>; }
+
+hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st) {
+  const void *f;
+  if (G == kFlat) {
+    if (D == 2) f = J ? recheck_fn<kFlat, 2, true>() : recheck_fn<kFlat, 2, false>();
+    else f = J ? recheck_fn<kFlat, 3, true>() : recheck_fn<kFlat, 3, false>();
+  } else {
+    if (D == 2) f = J ? recheck_fn<kSphere, 2, true>() : recheck_fn<kSphere, 2, false>();
+    else f = J ? recheck_fn<kSphere, 3, true>() : recheck_fn<kSphere, 3, false>();
+  }
+  Params Pc = P;
+  void *args[] = {&Pc};
+  return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st);
+}
+
+hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                        unsigned long long *dst_c, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  merge_out<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src_s, src_c, dst_s, dst_c, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
+                            hipStream_t st) {
+  eval_math<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(fn, n, a, b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st) {
+  const int64_t n = bytes / 4;
+  if (n <= 0) return hipSuccess;
+  zero_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>((int *)p, n);
+  return hipGetLastError();
+}
+
+}  // namespace sfk

@@ -1,0 +1,159 @@
+"""ctypes binding of libsfmi355x.so (include/sfmi355x.h).
+
+The library is built in-tree (structurefunctions.jl_amd/libsfmi355x.so).  There is no fallback:
+if it cannot be loaded, every entry point raises BackendError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .errors import ArgumentError, BackendError, DimensionMismatch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libsfmi355x.so")
+
+SF_OK, SF_E_ARG, SF_E_DIM, SF_E_COUNT, SF_E_DEVICE, SF_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+PREC_FP64, PREC_MIXED = 0, 1
+
+EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
+            "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_check", "sf_eval_math",
+            "sf_set_tuning")
+
+
+class SfProblem(ctypes.Structure):
+    _fields_ = [
+        ("D", ctypes.c_int32), ("W", ctypes.c_int32), ("N", ctypes.c_int64),
+        ("x", ctypes.c_void_p), ("u", ctypes.c_void_p),
+        ("geometry", ctypes.c_int32), ("radius", ctypes.c_double),
+        ("op", ctypes.c_int32), ("nl", ctypes.c_int32), ("nt", ctypes.c_int32),
+        ("n_dist", ctypes.c_int32), ("dist_edges", ctypes.c_void_p),
+        ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
+        ("precision", ctypes.c_int32), ("count_bits", ctypes.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the library once; raise BackendError if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BackendError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C csrc")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+    L.sf_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.sf_create.restype = ctypes.c_int
+    L.sf_destroy.argtypes = [vp]
+    L.sf_destroy.restype = None
+    L.sf_last_error.argtypes = [vp]
+    L.sf_last_error.restype = ctypes.c_char_p
+    L.sf_hist.argtypes = [vp, ctypes.POINTER(SfProblem), dp, dp]
+    L.sf_hist.restype = ctypes.c_int
+    L.sf_hist_batch.argtypes = [vp, i32, ctypes.POINTER(SfProblem), dp, dp]
+    L.sf_hist_batch.restype = ctypes.c_int
+    L.sf_hist_device.argtypes = [vp, ctypes.POINTER(SfProblem), dp, dp, vp]
+    L.sf_hist_device.restype = ctypes.c_int
+    L.sf_hist_batch_device.argtypes = [vp, i32, ctypes.POINTER(SfProblem), dp, dp, vp]
+    L.sf_hist_batch_device.restype = ctypes.c_int
+    L.sf_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.sf_last_stats.restype = ctypes.c_int
+    L.sf_check.argtypes = [i32, ctypes.POINTER(SfProblem), ctypes.c_char_p, i32]
+    L.sf_check.restype = ctypes.c_int
+    L.sf_eval_math.argtypes = [vp, i32, i64, dp, dp, dp]
+    L.sf_eval_math.restype = ctypes.c_int
+    L.sf_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.sf_set_tuning.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def raise_for(code: int, msg: str):
+    if code == SF_OK:
+        return
+    if code == SF_E_DIM:
+        raise DimensionMismatch(msg)
+    if code in (SF_E_ARG, SF_E_COUNT, SF_E_UNSUPPORTED):
+        raise ArgumentError(msg)
+    raise BackendError(msg or f"libsfmi355x error {code}")
+
+
+class Context:
+    """One device context (sf_create); not thread-safe, one call at a time."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.sf_create(int(device), ctypes.byref(h))
+        if rc != SF_OK:
+            raise BackendError(f"sf_create(device={device}) failed with code {rc} (no usable GPU?)")
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self) -> str:
+        m = lib().sf_last_error(self._h)
+        return m.decode() if m else ""
+
+    def check(self, rc: int):
+        if rc != SF_OK:
+            raise_for(rc, self.error())
+
+    def stats(self):
+        out = (ctypes.c_int64 * 8)()
+        lib().sf_last_stats(self._h, out)
+        return {"pairs": out[0], "rebinned": out[1], "items": out[2], "blocks": out[3],
+                "hist_ns": out[4], "recheck_ns": out[5], "call_ns": out[6], "block": out[7]}
+
+    def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0):
+        self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu)))
+
+    def eval_math(self, fn: int, a, b=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = len(a)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros(2 * n if fn <= 1 else n)
+        rc = lib().sf_eval_math(self._h, fn, n, a.ctypes.data, None if bb is None else bb.ctypes.data,
+                                out.ctypes.data)
+        self.check(rc)
+        return out.reshape(n, 2) if fn <= 1 else out
+
+    def hist(self, problems, sums, counts):
+        """Host-pointer batch call; sums/counts are contiguous arrays (n_slices * ncell)."""
+        arr = (SfProblem * len(problems))(*problems)
+        rc = lib().sf_hist_batch(self._h, len(problems), arr, sums.ctypes.data, counts.ctypes.data)
+        self.check(rc)
+
+    def hist_device(self, problems, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0):
+        """Device-pointer batch call (x/u in each problem and the outputs are device addresses)."""
+        arr = (SfProblem * len(problems))(*problems)
+        rc = lib().sf_hist_batch_device(self._h, len(problems), arr, ctypes.c_void_p(d_sums_ptr),
+                                        ctypes.c_void_p(d_counts_ptr),
+                                        ctypes.c_void_p(stream_ptr) if stream_ptr else None)
+        self.check(rc)
+
+
+def check_problems(problems) -> tuple[int, str]:
+    """sf_check: validation without a device."""
+    arr = (SfProblem * len(problems))(*problems)
+    buf = ctypes.create_string_buffer(512)
+    rc = lib().sf_check(len(problems), arr, buf, 512)
+    return rc, buf.value.decode()

@@ -1,0 +1,119 @@
+"""ctypes wrapper of the CPU oracle (oracle/libsforacle.so) for tests and bench.py's CPU baseline.
+
+The oracle is test infrastructure: a restatement of the reference's SerialBackend, used only to
+check the gfx950 results and to time a CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PATH = os.path.join(ROOT, "oracle", "libsforacle.so")
+
+GEOM = {"flat": 0, "deg": 1, "rad": 2}
+
+
+class Prob(ctypes.Structure):
+    _fields_ = [
+        ("D", ctypes.c_int32), ("W", ctypes.c_int32), ("N", ctypes.c_int64),
+        ("x", ctypes.c_void_p), ("u", ctypes.c_void_p),
+        ("geometry", ctypes.c_int32), ("radius", ctypes.c_double),
+        ("op", ctypes.c_int32), ("nl", ctypes.c_int32), ("nt", ctypes.c_int32),
+        ("n_dist", ctypes.c_int32), ("dist_edges", ctypes.c_void_p),
+        ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(PATH)
+        L.sfo_hist.argtypes = [ctypes.POINTER(Prob), ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                               ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_void_p]
+        L.sfo_hist.restype = ctypes.c_int
+        L.sfo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.sfo_atan2.restype = ctypes.c_double
+        L.sfo_sincosd.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.sfo_sincos.argtypes = L.sfo_sincosd.argtypes
+        L.sfo_pair.argtypes = [ctypes.POINTER(Prob), ctypes.c_int64, ctypes.c_int64,
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.sfo_pair.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):
+    keep = []
+    xc = np.asfortranarray(x, dtype=np.float64)
+    uc = np.asfortranarray(u, dtype=np.float64)
+    de = np.ascontiguousarray(edges, dtype=np.float64)
+    keep += [xc, uc, de]
+    p = Prob()
+    p.D, p.W, p.N = uc.shape[0], xc.shape[0], xc.shape[1]
+    p.x, p.u = xc.ctypes.data, uc.ctypes.data
+    p.geometry = GEOM[geometry] if isinstance(geometry, str) else geometry
+    p.radius = radius
+    p.op, p.nl, p.nt = op
+    p.n_dist, p.dist_edges = len(de) - 1, de.ctypes.data
+    if vedges is None:
+        p.n_val, p.val_edges = 0, None
+    else:
+        ve = np.ascontiguousarray(vedges, dtype=np.float64)
+        keep.append(ve)
+        p.n_val, p.val_edges = len(ve) - 1, ve.ctypes.data
+    return p, keep
+
+
+def hist(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, nthreads=1,
+         compensated=True, rows=None, with_mag=False):
+    """Oracle histogram: returns (sums, counts, abs_sums[, mag_sums]); joint results are
+    (n_dist, n_val).  abs_sums = sum |value|, mag_sums = sum ||du||^order per bin."""
+    p, keep = make(x, u, edges, op, geometry, radius, vedges)
+    nd = p.n_dist
+    nv = p.n_val
+    ncell = nd * max(nv, 1)
+    s = np.zeros(ncell)
+    c = np.zeros(ncell, dtype=np.uint64)
+    a = np.zeros(ncell)
+    g = np.zeros(ncell)
+    i0, i1, st = (0, p.N, 1) if rows is None else rows
+    rc = lib().sfo_hist(ctypes.byref(p), nthreads, 1 if compensated else 0, i0, i1, st,
+                        s.ctypes.data, c.ctypes.data, a.ctypes.data, g.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the problem (code {rc})")
+    out = [s, c, a, g]
+    if nv:
+        out = [v.reshape((nd, nv), order="F") for v in out]
+    return tuple(out) if with_mag else tuple(out[:3])
+
+
+def pair(x, u, edges, i, j, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):
+    p, keep = make(x, u, edges, op, geometry, radius, vedges)
+    d, v = ctypes.c_double(), ctypes.c_double()
+    db, vb = ctypes.c_int64(), ctypes.c_int64()
+    ok = lib().sfo_pair(ctypes.byref(p), i, j, ctypes.byref(d), ctypes.byref(v), ctypes.byref(db), ctypes.byref(vb))
+    if ok < 0:
+        raise ValueError("bad pair")
+    return bool(ok), d.value, v.value, db.value, vb.value
+
+
+def sincosd(x):
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().sfo_sincosd(x, ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def sincos(x):
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().sfo_sincos(x, ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
+
+
+def atan2(y, x):
+    return lib().sfo_atan2(y, x)
