@@ -25,15 +25,16 @@ namespace sfk {
 // kernel launchers (sf_kernels.hip)
 hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *prep_slices,
                        int n_slices, int64_t max_n, double *rec64, float *rec32, hipStream_t st);
-hipError_t launch_hist(int G, int D, bool J, bool mixed, const Params &P, int grid, int block,
-                       size_t smem, hipStream_t st);
+hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Params &P, int grid,
+                       int block, size_t smem, hipStream_t st);
 hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st);
 hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, double *dst_s,
                         unsigned long long *dst_c, int64_t n, hipStream_t st);
 hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
-int hist_occupancy(int G, int D, bool J, bool mixed, int block, size_t smem);
+int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, int block, size_t smem);
+int hist_opk(const Params &P);
 struct PrepSliceHost {
   const double *x, *u;
   int64_t N, pt_off;
@@ -154,7 +155,7 @@ struct sf_ctx {
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool ev_pending = false;
-  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0;
+  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
   // device buffers
   DevBuf<double> d_x, d_u;
   DevBuf<double> rec64;
@@ -337,7 +338,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
     }
   }
   // margins: relative 2^-18 (64 ulp) plus an absolute term for the hi/lo split of the inputs
-  const long double drel = ldexpl(1.0L, -18);
+  const long double drel = ldexpl(1.0L, -20);
   const long double e1 = flat ? ldexpl(1.0L, -44) * std::max((long double)xmax, 1e-300L) : ldexpl(1.0L, -44);
   std::vector<float> lo(nE), hi(nE);
   for (int k = 0; k < nE; ++k) {
@@ -507,11 +508,14 @@ extern "C" int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a,
   return SF_OK;
 }
 
-extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu) {
+extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu,
+                             int32_t variant) {
   if (!ctx) return SF_E_ARG;
-  if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512)
-    return ctx->fail(SF_E_ARG, "block must be 0, 64, 128, 256 or 512");
+  if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+    return ctx->fail(SF_E_ARG, "block must be 0, 64, 128, 256, 512 or 1024");
   if (j_chunk < 0 || blocks_per_cu < 0) return ctx->fail(SF_E_ARG, "tuning values must be >= 0");
+  if (variant < 0 || variant > 2) return ctx->fail(SF_E_ARG, "variant must be 0, 1 or 2");
+  ctx->tune_variant = variant;
   ctx->tune_block = block;
   ctx->tune_jchunk = j_chunk;
   ctx->tune_bpc = blocks_per_cu;
@@ -532,8 +536,8 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   const bool J = p0.n_val > 0;
   const bool mixed = p0.precision == SF_PREC_MIXED;
   const int ncell = p0.n_dist * (J ? p0.n_val : 1);
-  const int S64 = flat ? (D == 2 ? 4 : 6) : 12;
-  const int F32 = (flat && D == 2) ? 8 : 12;
+  const int S64 = rec_d64(G, D);
+  const int F32 = rec_f32(G, D);
 
   // ---- records
   int64_t total_pts = 0, max_n = 0, total_pairs = 0;
@@ -591,15 +595,32 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   }
   if (vedges.empty()) { vedges.push_back(0.0); vedges32.push_back(0.0f); }
 
-  // ---- launch geometry
-  int block = ctx->tune_block ? ctx->tune_block : 256;
-  const int stage_bytes = mixed ? block * F32 * 4 : block * S64 * 8;
-  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes);
+  // ---- launch geometry: one histogram per workgroup; large joint histograms get 1024-thread
+  // workgroups (one per CU, 16 waves share the table), small 1D histograms 256-thread workgroups
+  // with per-lane interleaved copies (several workgroups per CU)
+  int block = ctx->tune_block ? ctx->tune_block : (J ? 1024 : 256);
+  const int tj = std::min(block, 256);
+  // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
+  // (binary32 kernel only); 0 picks the default
+  const bool sload = mixed && (ctx->tune_variant == 2 || ctx->tune_variant == 0);
+  const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
+  const int stage_bytes = sload ? 0 : tj * rec_bytes;
+  int copies = 1;
+  if (!J) {
+    copies = 64;
+    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies).total * 2 > ctx->max_smem)
+      copies >>= 1;
+  }
+  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies);
   if ((size_t)L.total > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
                      ncell, L.total, ctx->max_smem);
-  int bpc = hist_occupancy(G, D, J, mixed, block, (size_t)L.total);
+  Params Pk{};
+  Pk.op = p0.op;
+  Pk.nl = p0.nl;
+  Pk.nt = p0.nt;
+  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, block, (size_t)L.total);
   if (bpc < 1) bpc = 1;
   if (ctx->tune_bpc) bpc = std::min(bpc, ctx->tune_bpc);
   const int max_blocks = bpc * ctx->n_cu;
@@ -681,6 +702,9 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   P.pending_cap = cap;
   P.stats = nullptr;
   P.ncell_max = ncell_max;
+  P.tj = tj;
+  P.copies = copies;
+  P.stage_bytes = stage_bytes;
   P.n_dist = p0.n_dist;
   P.n_val = p0.n_val;
   P.op = p0.op;
@@ -688,8 +712,8 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   P.nt = p0.nt;
   P.order = op_order(p0);
   P.radius = (p0.geometry == SF_GEOM_SPHERE_DEG) ? p0.radius : 1.0;
-  // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + 4 / sin(sigma))
-  P.vcoef = (float)ldexp((flat ? 64.0 : 128.0) * std::max(1, P.order), -24);
+  // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + c^2 / sin(sigma))
+  P.vcoef = (float)ldexp(16.0 * std::max(1, P.order), -24);
 
   for (int attempt = 0; attempt < 2; ++attempt) {
     const bool use_mixed = mixed && attempt == 0;
@@ -697,12 +721,14 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
     HIPCHK(launch_zero(ctx->acc_c.p, sizeof(unsigned long long) * (int64_t)n * ncell, st));
     HIPCHK(launch_zero(ctx->counter.p, sizeof(int) * 4, st));
     HIPCHK(launch_zero(ctx->pcount.p, sizeof(unsigned long long) * 4, st));
-    const int stage = use_mixed ? block * F32 * 4 : block * S64 * 8;
-    const SmemLayout L2 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage);
+    const bool use_sload = use_mixed && sload;
+    const int stage = use_sload ? 0 : tj * (use_mixed ? F32 * 4 : S64 * 8);
+    P.stage_bytes = stage;
+    const SmemLayout L2 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage, copies);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(launch_hist(G, D, J, use_mixed, P, grid, block, (size_t)L2.total, st));
+    HIPCHK(launch_hist(G, D, J, use_mixed, use_sload, P, grid, block, (size_t)L2.total, st));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    if (use_mixed) HIPCHK(launch_recheck(G, D, J, P, 4 * ctx->n_cu, st));
+    if (use_mixed) HIPCHK(launch_recheck(G, D, J, P, ctx->n_cu, st));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
     if (!use_mixed) break;
     unsigned long long pc = 0;

@@ -24,9 +24,11 @@ enum Geom : int { kFlat = 0, kSphere = 1 };
 //     sphere D  : ph0..2 pl0..2 Uh0..2 Ul0..2 |u| urh url 0                  (16 floats)
 //                 U = u_e E + u_n N is the tangent velocity in R^3, ur the radial component
 //                 (D = 3), P the unit position; |u| is an upper bound of the velocity norm.
+inline constexpr __host__ __device__ int rec_d64(int G, int D) { return (G == kFlat) ? ((D == 2) ? 4 : 6) : 12; }
+inline constexpr __host__ __device__ int rec_f32(int G, int D) { return (G == kFlat && D == 2) ? 12 : 16; }
 template <int G, int D> struct Rec {
-  static constexpr int d64 = (G == kFlat) ? ((D == 2) ? 4 : 6) : 12;
-  static constexpr int f32 = (G == kFlat && D == 2) ? 12 : 16;
+  static constexpr int d64 = rec_d64(G, D);
+  static constexpr int f32 = rec_f32(G, D);
 };
 
 // Distance-key cell table entry.  The positive key axis (flat: r^2; sphere: chord^2 for the
@@ -87,6 +89,10 @@ struct Params {
   int64_t pending_cap;
   unsigned long long *stats;  // [0] pairs visited by exact kernel recheck
   int32_t ncell_max;       // largest table of any slice (LDS sizing)
+  int32_t tj;              // column records staged in LDS per chunk
+  int32_t copies;          // 1D: per-lane histogram copies (power of two); joint: 1
+  int32_t vstride;         // joint: LDS row stride of a distance bin (odd, >= n_val); 1D: 0
+  int32_t stage_bytes;     // LDS bytes of the column stage
   int32_t n_dist, n_val;
   int32_t op, nl, nt;
   int32_t order;           // operator order (value-error bound exponent)
@@ -100,10 +106,15 @@ struct SmemLayout {
   int32_t hs, hc, tab, dE, vE, vE32, stage, total;  // byte offsets
 };
 
+// LDS histogram slots: joint histograms are stored value-bin fastest with an odd row stride per
+// distance bin (lanes of one wave usually share a distance bin but not a value bin, so their
+// atomics then fall in different banks); 1D histograms keep one copy per lane group.
+inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
+
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
-                                                  int stage_bytes) {
+                                                  int stage_bytes, int copies) {
   SmemLayout L;
-  const int nh = n_dist * (n_val > 0 ? n_val : 1);
+  const int nh = (n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies;
   int o = 0;
   L.hs = o; o += 8 * nh;
   L.hc = o; o += 4 * ((nh + 3) & ~3);

@@ -222,8 +222,10 @@ DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
 }
 
 // Full exact evaluation of pair (i, j) of slice s from binary64 records.  Returns the output cell
-// (0-based within the slice) or -1 if the pair is not counted; writes the value.
-template <int G, int D, bool J>
+// (0-based within the slice) or -1 if the pair is not counted; writes the value.  OPK fixes the
+// operator at compile time (1: {1,0}, 2: {2,0}) with the same operation sequence as the generic
+// functor; 0 dispatches on P.op at run time.
+template <int G, int D, bool J, int OPK = 0>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
                    const Cell *tab, const double *dE, const double *vE, double *val) {
   const int nEd = P.n_dist + 1;
@@ -239,7 +241,18 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     const double r2 = sdot<D>(dx, dx);
     db = dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
-    v = sf_raw64<D>(P.op, P.nl, P.nt, du, dx, r2);
+    if constexpr (OPK == 2) {
+      const double p = sdot<D>(du, dx);
+      v = p * p / r2;
+    } else if constexpr (OPK == 1) {
+      const double sq = __builtin_sqrt(r2);
+      double rh[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) rh[k] = dx[k] / sq;
+      v = 1.0 * sdot<D>(du, rh);
+    } else {
+      v = sf_raw64<D>(P.op, P.nl, P.nt, du, dx, r2);
+    }
   } else {
     const SphereFrame f = sphere_frame64(A, B, P.radius);
     db = dist_bin64(f.distance, tab, dE, nEd, s);
@@ -247,7 +260,13 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     double du[3];
     sphere_delta64<D>(f, A, B, du);
     const double rh[3] = {1.0, 0.0, 0.0};
-    v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+    if constexpr (OPK == 1) v = 1.0 * sdot<D>(du, rh);
+    else if constexpr (OPK == 2) {
+      const double L = sdot<D>(du, rh);
+      v = 1.0 * (L * L);
+    } else {
+      v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+    }
   }
   *val = v;
   if (!J) return db - 1;
@@ -354,12 +373,20 @@ struct Smem {
   float *vE32;
   char *stage;
   int ncell_h;
+  int copies;
+  bool joint;
+  int nd, vstride, nslots;
 };
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies);
   Smem m;
   m.ncell_h = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
+  m.copies = P.copies;
+  m.joint = P.n_val > 0;
+  m.nd = P.n_dist;
+  m.vstride = P.n_val > 0 ? joint_stride(P.n_val) : 0;
+  m.nslots = P.n_val > 0 ? P.n_dist * m.vstride : P.n_dist * P.copies;
   m.hs = (double *)(base + L.hs);
   m.hc = (unsigned *)(base + L.hc);
   m.tab = (Cell *)(base + L.tab);
@@ -371,20 +398,40 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
 }
 
 DEV void zero_hist(const Smem &m) {
-  for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
+  for (int c = threadIdx.x; c < m.nslots; c += blockDim.x) {
     m.hs[c] = 0.0;
     m.hc[c] = 0u;
   }
 }
 
+// LDS slot of output cell c = db0 + n_dist * vb0 (0-based bins) for this lane
+DEV int hslot(const Smem &m, int db0, int vb0) {
+  if (m.joint) return vb0 + m.vstride * db0;
+  return db0 * m.copies + (__lane_id() & (m.copies - 1));
+}
+
 DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
   for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
-    const unsigned n = m.hc[c];
+    unsigned long long n = 0;
+    double v = 0.0;
+    if (m.joint) {
+      const int db0 = c % m.nd, vb0 = c / m.nd;
+      const int h = vb0 + m.vstride * db0;
+      n = m.hc[h];
+      v = m.hs[h];
+      m.hc[h] = 0u;
+      m.hs[h] = 0.0;
+    } else {
+      for (int k = 0; k < m.copies; ++k) {
+        n += m.hc[c * m.copies + k];
+        v += m.hs[c * m.copies + k];
+        m.hc[c * m.copies + k] = 0u;
+        m.hs[c * m.copies + k] = 0.0;
+      }
+    }
     if (n) {
-      atomicAdd(&P.counts[s.out_off + c], (unsigned long long)n);
-      atomicAdd(&P.sums[s.out_off + c], m.hs[c]);
-      m.hs[c] = 0.0;
-      m.hc[c] = 0u;
+      atomicAdd(&P.counts[s.out_off + c], n);
+      atomicAdd(&P.sums[s.out_off + c], v);
     }
   }
 }
@@ -405,7 +452,7 @@ DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32
 // ---------------------------------------------------------------------------------------------
 // hist64: binary64 exact pair arithmetic for every pair
 // ---------------------------------------------------------------------------------------------
-template <int G, int D, bool J, bool MASK>
+template <int G, int D, bool J, int OPK, bool MASK>
 DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i,
                      int jb, int cnt) {
   constexpr int S = Rec<G, D>::d64;
@@ -414,22 +461,28 @@ DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const doubl
     const int j = jb + jj;
     if (MASK && j <= i) continue;
     double rj[S];
+    const double2 *st2 = (const double2 *)st;
 #pragma unroll
-    for (int k = 0; k < S; ++k) rj[k] = st[jj * S + k];
+    for (int k = 0; k < S / 2; ++k) {
+      const double2 q = st2[jj * (S / 2) + k];
+      rj[2 * k] = q.x;
+      rj[2 * k + 1] = q.y;
+    }
     double v;
-    const int c = exact_cell<G, D, J>(P, s, ri, rj, m.tab, m.dE, m.vE, &v);
+    const int c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, &v);
     if (c >= 0) {
-      atomicAdd(&m.hs[c], v);
-      atomicAdd(&m.hc[c], 1u);
+      const int h = hslot(m, c % P.n_dist, c / P.n_dist);
+      atomicAdd(&m.hs[h], v);
+      atomicAdd(&m.hc[h], 1u);
     }
   }
 }
 
-template <int G, int D, bool J>
-__global__ void __launch_bounds__(512) hist64(Params P) {
+template <int G, int D, bool J, int OPK>
+__global__ void __launch_bounds__(1024) hist64(Params P, const double *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
-  const Smem m = carve(smem_raw, P, 0);
+  const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
   const int B = blockDim.x;
   int cur = -1;
@@ -461,17 +514,17 @@ __global__ void __launch_bounds__(512) hist64(Params P) {
     const bool iv = i < s.N;
     double ri[S];
 #pragma unroll
-    for (int k = 0; k < S; ++k) ri[k] = iv ? P.rec64[(s.pt_off + i) * S + k] : 0.0;
-    for (int jb = w.j0; jb < w.j1; jb += B) {
-      const int cnt = min(B, w.j1 - jb);
+    for (int k = 0; k < S; ++k) ri[k] = iv ? rec[(s.pt_off + i) * S + k] : 0.0;
+    for (int jb = w.j0; jb < w.j1; jb += P.tj) {
+      const int cnt = min(P.tj, w.j1 - jb);
       __syncthreads();
-      const double *src = P.rec64 + (s.pt_off + jb) * S;
-      double *dst = (double *)m.stage;
-      for (int k = threadIdx.x; k < cnt * S; k += B) dst[k] = src[k];
+      const double2 *src = (const double2 *)(rec + (s.pt_off + jb) * S);
+      double2 *dst = (double2 *)m.stage;
+      for (int k = threadIdx.x; k < cnt * (S / 2); k += B) dst[k] = src[k];
       __syncthreads();
       if (iv) {
-        if (jb < w.i0 + B) run_chunk64<G, D, J, true>(P, s, m, ri, i, jb, cnt);
-        else run_chunk64<G, D, J, false>(P, s, m, ri, i, jb, cnt);
+        if (jb < w.i0 + B) run_chunk64<G, D, J, OPK, true>(P, s, m, ri, i, jb, cnt);
+        else run_chunk64<G, D, J, OPK, false>(P, s, m, ri, i, jb, cnt);
       }
     }
   }
@@ -546,160 +599,180 @@ DEV float op_value32(int op, int nl, int nt, int D, float L, float T, float tn2,
   }
 }
 
-template <int G, int D, bool J, bool MASK>
-DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
-                     int jb, int cnt, int slice_id) {
-  constexpr int F = Rec<G, D>::f32;
-  const float *st = (const float *)m.stage;
-  const int nb = P.n_dist;
-  for (int jj = 0; jj < cnt; ++jj) {
-    const int j = jb + jj;
-    if (MASK && j <= i) continue;
-    float rj[F];
+// Operator class fixed at compile time: 0 = any operator (runtime switch), 1 = du_L
+// (ProjectedStructureFunctionType{1,0}), 2 = du_L^2 ({2,0}).
+template <int OPK>
+DEV float op32(const Params &P, int D, float L, float T, float tn2, float n2) {
+  if constexpr (OPK == 1) return L;
+  else if constexpr (OPK == 2) return L * L;
+  else return op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
+}
+
+// (|u_i| + |u_j|)^order without a loop (order 0..3)
+template <int OPK>
+DEV float pow_order(const Params &P, float x) {
+  if constexpr (OPK == 1) return x;
+  else if constexpr (OPK == 2) return x * x;
+  else {
+    const float x2 = x * x;
+    return P.order == 0 ? 1.0f : (P.order == 1 ? x : (P.order == 2 ? x2 : x2 * x));
+  }
+}
+
+template <int F>
+DEV void load_rec(const float4 *col, int idx, float *r) {
 #pragma unroll
-    for (int k = 0; k < F; ++k) rj[k] = st[jj * F + k];
+  for (int k = 0; k < F / 4; ++k) {
+    const float4 q = col[idx * (F / 4) + k];
+    r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
+  }
+}
+
+// Pairs (i, j) for j in [j_begin, j_end); column j's record is col[(j - col_j0) * F/4 ...],
+// either the LDS stage or (scalar-load variant) the record array itself.  The body is written
+// without early exits so that the next column's record can be requested mid-iteration, after the
+// only LDS read whose result the iteration waits for (the distance-cell table).
+template <int G, int D, bool J, int OPK, bool MASK>
+DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
+                     int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
+  constexpr int F = Rec<G, D>::f32;
+  const int nb = P.n_dist;
+  constexpr bool needT = (OPK == 0);
+  if (j_begin >= j_end) return;
+  float rj[F];
+  load_rec<F>(col, j_begin - col_j0, rj);
+  for (int j = j_begin; j < j_end; ++j) {
+    bool live = !MASK || j > i;
     bool amb = false;
     float key, L = 0.f, T = 0.f, tn2 = 0.f, n2 = 0.f, amp = 1.0f, umax;
-    if (G == kFlat) {
-      float dx[D], du[D];
+    int db;
+    float rn[F];
+    // ---- distance key and certified bin
+    float dx[3], du[3], d[3];
+    if constexpr (G == kFlat) {
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
-        du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
-      }
+      for (int k = 0; k < D; ++k) dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
       key = dx[0] * dx[0];
 #pragma unroll
       for (int k = 1; k < D; ++k) key = __builtin_fmaf(dx[k], dx[k], key);
       umax = ri[4 * D] + rj[4 * D];
-      // distance bin
-      const Cell c = m.tab[cell32(key, s)];
-      amb = (key >= c.lo) && (key <= c.hi);
-      const int db = (key > c.hi) ? c.blo + 1 : c.blo;
-      if (!amb && (db < 1 || db > nb)) continue;
-      float p = du[0] * dx[0];
-#pragma unroll
-      for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
-      n2 = du[0] * du[0];
-#pragma unroll
-      for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
-      const float rinv = frsq(key);
-      L = p * rinv;
-      // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
-      // formed from the cross product so that it does not cancel when du is nearly longitudinal
-      const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
-      if (D == 2) {
-        T = cz * rinv;
-        tn2 = T * T;
-      } else {
-        T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
-        const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
-        const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
-        tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
-      }
-      amb = amb || !(key > 1e-30f);
-      const float v = op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
-      int cell = db - 1;
-      if (J) {
-        float up = 1.0f;
-        for (int e = 0; e < P.order; ++e) up *= umax;
-        const float dv = P.vcoef * up * amp;
-        const int vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
-        if (!amb && (vb < 1 || vb > P.n_val)) continue;
-        cell += nb * (vb - 1);
-      } else if (!(__builtin_fabsf(v) < __builtin_inff())) {
-        amb = true;
-      }
-      const unsigned long long mask = __ballot(amb);
-      if (mask) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
-        if (amb) {
-          const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-          if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
-          continue;
-        }
-      }
-      atomicAdd(&m.hs[cell], (double)v);
-      atomicAdd(&m.hc[cell], 1u);
     } else {
-      // sphere: records ph[3] pl[3] Uh[3] Ul[3] |u| urh url 0
-      float d[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) d[k] = (rj[k] - ri[k]) + (rj[3 + k] - ri[3 + k]);
       key = __builtin_fmaf(d[2], d[2], __builtin_fmaf(d[1], d[1], d[0] * d[0]));
       umax = ri[12] + rj[12];
+    }
+    {
       const Cell c = m.tab[cell32(key, s)];
       amb = (key >= c.lo) && (key <= c.hi);
-      const int db = (key > c.hi) ? c.blo + 1 : c.blo;
-      if (!amb && (db < 1 || db > nb)) continue;
-      // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
-      //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
-      //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
-      //   du_T = w.(U_B - U_A)/sin
-      const float dp = -0.5f * key;
-      float tB[3], w[3], dU[3], sp[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
-        dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
-        sp[k] = ri[k] + rj[k];
-      }
-      w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
-      w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
-      w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
-      const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
-      amb = amb || !(s2 > 1e-30f);
-      const float inv_s = frsq(s2);
-      const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
-      const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
-      L = __builtin_fmaf(dp, sU, tdU) * inv_s;
-      T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
-      tn2 = T * T;
-      if (D == 3) {
-        const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
-        tn2 = __builtin_fmaf(dw, dw, tn2);
-      }
-      n2 = __builtin_fmaf(L, L, tn2);
-      amp = __builtin_fmaf(4.0f, inv_s, 1.0f);
-      const float v = op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
-      int cell = db - 1;
-      if (J) {
-        float up = 1.0f;
-        for (int e = 0; e < P.order; ++e) up *= umax;
-        const float dv = P.vcoef * up * amp;
-        const int vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
-        if (!amb && (vb < 1 || vb > P.n_val)) continue;
-        cell += nb * (vb - 1);
-      } else if (!(__builtin_fabsf(v) < __builtin_inff())) {
-        amb = true;
-      }
-      const unsigned long long mask = __ballot(amb);
-      if (mask) {
-        const int lane = __lane_id();
-        const int leader = __ffsll((long long)mask) - 1;
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
-        base = __shfl(base, leader);
-        if (amb) {
-          const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-          if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
-          continue;
-        }
-      }
-      atomicAdd(&m.hs[cell], (double)v);
-      atomicAdd(&m.hc[cell], 1u);
+      db = (key > c.hi) ? c.blo + 1 : c.blo;
     }
+    if (j + 1 < j_end) load_rec<F>(col, j + 1 - col_j0, rn);
+    live = live && (amb || (db >= 1 && db <= nb));
+    float v = 0.f;
+    int vb = 1;
+    if (live) {
+      if constexpr (G == kFlat) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
+        float p = du[0] * dx[0];
+#pragma unroll
+        for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
+        const float rinv = frsq(key);
+        L = p * rinv;
+        if constexpr (needT) {
+          n2 = du[0] * du[0];
+#pragma unroll
+          for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
+          // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
+          // formed from the cross product so that it does not cancel when du is nearly longitudinal
+          const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
+          if constexpr (D == 2) {
+            T = cz * rinv;
+            tn2 = T * T;
+          } else {
+            T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
+            const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
+            const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
+            tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
+          }
+        }
+        amb = amb || !(key > 1e-30f);
+      } else {
+        // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
+        //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
+        //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
+        //   du_T = w.(U_B - U_A)/sin
+        const float dp = -0.5f * key;
+        float tB[3], w[3], dU[3], sp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
+          dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
+          sp[k] = ri[k] + rj[k];
+        }
+        w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
+        w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
+        w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
+        const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
+        amb = amb || !(s2 > 1e-30f);
+        const float inv_s = frsq(s2);
+        const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
+        const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
+        L = __builtin_fmaf(dp, sU, tdU) * inv_s;
+        if constexpr (needT) {
+          T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
+          tn2 = T * T;
+          if constexpr (D == 3) {
+            const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
+            tn2 = __builtin_fmaf(dw, dw, tn2);
+          }
+          n2 = __builtin_fmaf(L, L, tn2);
+        }
+        // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)) (cancellation in t_B
+        // and in the transport term near antipodal pairs)
+        amp = __builtin_fmaf(key, inv_s, 1.0f);
+      }
+      v = op32<OPK>(P, D, L, T, tn2, n2);
+      if constexpr (J) {
+        const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+        vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
+        live = amb || (vb >= 1 && vb <= P.n_val);
+      } else {
+        amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
+      }
+    }
+    const bool defer = live && amb;
+    const unsigned long long mask = __ballot(defer);
+    if (mask) {
+      const int lane = __lane_id();
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (defer) {
+        const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+        if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
+      }
+    }
+    if (live && !amb) {
+      const int h = hslot(m, db - 1, vb - 1);
+      atomicAdd(&m.hs[h], (double)v);
+      atomicAdd(&m.hc[h], 1u);
+    }
+#pragma unroll
+    for (int k = 0; k < F; ++k) rj[k] = rn[k];
   }
 }
 
-template <int G, int D, bool J>
-__global__ void __launch_bounds__(512) hist32(Params P) {
+// SLOAD: column records are read straight from global memory with wave-uniform addresses, which
+// the compiler turns into scalar loads (SGPR operands, scalar cache) -- no LDS staging, no
+// barriers inside an item.  Otherwise columns are staged through LDS in chunks of P.tj.
+template <int G, int D, bool J, int OPK, bool SLOAD>
+__global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
-  const Smem m = carve(smem_raw, P, 0);
+  const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
   const int B = blockDim.x;
   int cur = -1;
@@ -723,6 +796,7 @@ __global__ void __launch_bounds__(512) hist32(Params P) {
         cur = w.slice;
         s = P.slices[cur];
         load_tables(m, P, s, true);
+        if (SLOAD) __syncthreads();
       }
       since_flush = 0;
     }
@@ -731,17 +805,27 @@ __global__ void __launch_bounds__(512) hist32(Params P) {
     const bool iv = i < s.N;
     float ri[F];
 #pragma unroll
-    for (int k = 0; k < F; ++k) ri[k] = iv ? P.rec32[(s.pt_off + i) * F + k] : 0.0f;
-    for (int jb = w.j0; jb < w.j1; jb += B) {
-      const int cnt = min(B, w.j1 - jb);
-      __syncthreads();
-      const float4 *src = (const float4 *)(P.rec32 + (s.pt_off + jb) * F);
-      float4 *dst = (float4 *)m.stage;
-      for (int k = threadIdx.x; k < cnt * (F / 4); k += B) dst[k] = src[k];
-      __syncthreads();
+    for (int k = 0; k < F; ++k) ri[k] = iv ? rec[(s.pt_off + i) * F + k] : 0.0f;
+    if constexpr (SLOAD) {
       if (iv) {
-        if (jb < w.i0 + B) run_chunk32<G, D, J, true>(P, s, m, ri, i, jb, cnt, cur);
-        else run_chunk32<G, D, J, false>(P, s, m, ri, i, jb, cnt, cur);
+        const float4 *col = (const float4 *)(rec + s.pt_off * F);
+        const int jm = min(w.j1, w.i0 + B);  // columns that can still be <= some row of the tile
+        if (w.j0 < jm) run_chunk32<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, jm, col, 0, cur);
+        if (jm < w.j1) run_chunk32<G, D, J, OPK, false>(P, s, m, ri, i, max(jm, w.j0), w.j1, col, 0, cur);
+      }
+    } else {
+      for (int jb = w.j0; jb < w.j1; jb += P.tj) {
+        const int cnt = min(P.tj, w.j1 - jb);
+        __syncthreads();
+        const float4 *src = (const float4 *)(rec + (s.pt_off + jb) * F);
+        float4 *dst = (float4 *)m.stage;
+        for (int k = threadIdx.x; k < cnt * (F / 4); k += B) dst[k] = src[k];
+        __syncthreads();
+        if (iv) {
+          const float4 *col = (const float4 *)m.stage;
+          if (jb < w.i0 + B) run_chunk32<G, D, J, OPK, true>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+          else run_chunk32<G, D, J, OPK, false>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+        }
       }
     }
   }
@@ -753,26 +837,56 @@ __global__ void __launch_bounds__(512) hist32(Params P) {
 // recheck: binary64 binning of queued pairs (global-memory tables, global atomics)
 // ---------------------------------------------------------------------------------------------
 template <int G, int D, bool J>
-__global__ void recheck(Params P) {
+__global__ void __launch_bounds__(256) recheck(Params P) {
+  // Each workgroup takes a contiguous range of the queue.  Entries of the range's first slice are
+  // accumulated in an LDS histogram flushed once; entries of other slices (the queue is filled in
+  // roughly slice order, so these are few) go straight to the global accumulators.
+  extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
+  const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
+  double *hs = (double *)smem_raw;
+  unsigned *hc = (unsigned *)(hs + nh);
+  for (int c = threadIdx.x; c < nh; c += blockDim.x) {
+    hs[c] = 0.0;
+    hc[c] = 0u;
+  }
   const unsigned long long n0 = *P.pending_count;
   const unsigned long long n = n0 < (unsigned long long)P.pending_cap ? n0 : (unsigned long long)P.pending_cap;
-  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (unsigned long long)gridDim.x * blockDim.x) {
-    const Pending q = P.pending[k];
-    const Slice s = P.slices[q.slice];
-    double A[S], Bv[S];
+  const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+  const unsigned long long beg = (unsigned long long)blockIdx.x * per;
+  const unsigned long long end = beg + per < n ? beg + per : n;
+  __syncthreads();
+  if (beg < end) {
+    const int home = P.pending[beg].slice;
+    for (unsigned long long k = beg + threadIdx.x; k < end; k += blockDim.x) {
+      const Pending q = P.pending[k];
+      const Slice s = P.slices[q.slice];
+      double A[S], Bv[S];
 #pragma unroll
-    for (int t = 0; t < S; ++t) {
-      A[t] = P.rec64[(s.pt_off + q.i) * S + t];
-      Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
+      for (int t = 0; t < S; ++t) {
+        A[t] = P.rec64[(s.pt_off + q.i) * S + t];
+        Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
+      }
+      double v;
+      const int c = exact_cell<G, D, J>(P, s, A, Bv, P.cells + s.tab_off, P.dedges + s.dedge_off,
+                                        P.vedges + s.vedge_off, &v);
+      if (c >= 0) {
+        if (q.slice == home) {
+          atomicAdd(&hs[c], v);
+          atomicAdd(&hc[c], 1u);
+        } else {
+          atomicAdd(&P.counts[s.out_off + c], 1ull);
+          atomicAdd(&P.sums[s.out_off + c], v);
+        }
+      }
     }
-    double v;
-    const int c = exact_cell<G, D, J>(P, s, A, Bv, P.cells + s.tab_off, P.dedges + s.dedge_off,
-                                      P.vedges + s.vedge_off, &v);
-    if (c >= 0) {
-      atomicAdd(&P.counts[s.out_off + c], 1ull);
-      atomicAdd(&P.sums[s.out_off + c], v);
+    __syncthreads();
+    const int64_t off = P.slices[home].out_off;
+    for (int c = threadIdx.x; c < nh; c += blockDim.x) {
+      if (hc[c]) {
+        atomicAdd(&P.counts[off + c], (unsigned long long)hc[c]);
+        atomicAdd(&P.sums[off + c], hs[c]);
+      }
     }
   }
 }
@@ -845,34 +959,53 @@ hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *ps, 
                 : prep_impl<kSphere, 3>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st);
 }
 
+template <int G, int D, bool J, bool SL>
+static const void *hist32_fn(int opk) {
+  if (opk == 1) return (const void *)&hist32<G, D, J, 1, SL>;
+  if (opk == 2) return (const void *)&hist32<G, D, J, 2, SL>;
+  return (const void *)&hist32<G, D, J, 0, SL>;
+}
+
 template <int G, int D, bool J>
-static const void *hist_fn(bool mixed) {
-  return mixed ? (const void *)&hist32<G, D, J> : (const void *)&hist64<G, D, J>;
+static const void *hist_fn(bool mixed, int opk, bool sload) {
+  if (mixed) return sload ? hist32_fn<G, D, J, true>(opk) : hist32_fn<G, D, J, false>(opk);
+  if (opk == 1) return (const void *)&hist64<G, D, J, 1>;
+  if (opk == 2) return (const void *)&hist64<G, D, J, 2>;
+  return (const void *)&hist64<G, D, J, 0>;
 }
 
-static const void *pick_hist(int G, int D, bool J, bool mixed) {
+static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl) {
   if (G == kFlat) {
-    if (D == 2) return J ? hist_fn<kFlat, 2, true>(mixed) : hist_fn<kFlat, 2, false>(mixed);
-    return J ? hist_fn<kFlat, 3, true>(mixed) : hist_fn<kFlat, 3, false>(mixed);
+    if (D == 2) return J ? hist_fn<kFlat, 2, true>(mixed, opk, sl) : hist_fn<kFlat, 2, false>(mixed, opk, sl);
+    return J ? hist_fn<kFlat, 3, true>(mixed, opk, sl) : hist_fn<kFlat, 3, false>(mixed, opk, sl);
   }
-  if (D == 2) return J ? hist_fn<kSphere, 2, true>(mixed) : hist_fn<kSphere, 2, false>(mixed);
-  return J ? hist_fn<kSphere, 3, true>(mixed) : hist_fn<kSphere, 3, false>(mixed);
+  if (D == 2) return J ? hist_fn<kSphere, 2, true>(mixed, opk, sl) : hist_fn<kSphere, 2, false>(mixed, opk, sl);
+  return J ? hist_fn<kSphere, 3, true>(mixed, opk, sl) : hist_fn<kSphere, 3, false>(mixed, opk, sl);
 }
 
-int hist_occupancy(int G, int D, bool J, bool mixed, int block, size_t smem) {
-  const void *f = pick_hist(G, D, J, mixed);
+static int opk_of(const Params &P) {
+  if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
+  if (P.op == 0 && P.nl == 2 && P.nt == 0) return 2;
+  return 0;
+}
+
+int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, int block, size_t smem) {
+  const void *f = pick_hist(G, D, J, mixed, opk, sload);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, smem) != hipSuccess) return 1;
   return nb;
 }
 
-hipError_t launch_hist(int G, int D, bool J, bool mixed, const Params &P, int grid, int block,
-                       size_t smem, hipStream_t st) {
-  const void *f = pick_hist(G, D, J, mixed);
+int hist_opk(const Params &P) { return opk_of(P); }
+
+hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Params &P, int grid,
+                       int block, size_t smem, hipStream_t st) {
+  const void *f = pick_hist(G, D, J, mixed, opk_of(P), sload);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   Params Pc = P;
-  void *args[] = {&Pc};
+  const void *rec = mixed ? (const void *)P.rec32 : (const void *)P.rec64;
+  void *args[] = {&Pc, &rec};
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, smem, st);
 }
 
@@ -890,7 +1023,10 @@ hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipSt
   }
   Params Pc = P;
   void *args[] = {&Pc};
-  return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st);
+  const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
+  const size_t smem = (size_t)nh * 12 + 16;
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  return hipLaunchKernel(f, dim3(grid), dim3(256), args, smem, st);
 }
 
 hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, double *dst_s,

@@ -66,7 +66,7 @@ def lib():
     L.sf_check.restype = ctypes.c_int
     L.sf_eval_math.argtypes = [vp, i32, i64, dp, dp, dp]
     L.sf_eval_math.restype = ctypes.c_int
-    L.sf_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.sf_set_tuning.argtypes = [vp, i32, i32, i32, i32]
     L.sf_set_tuning.restype = ctypes.c_int
     _lib = L
     return L
@@ -123,8 +123,8 @@ class Context:
         return {"pairs": out[0], "rebinned": out[1], "items": out[2], "blocks": out[3],
                 "hist_ns": out[4], "recheck_ns": out[5], "call_ns": out[6], "block": out[7]}
 
-    def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0):
-        self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu)))
+    def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0, variant=0):
+        self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu), int(variant)))
 
     def eval_math(self, fn: int, a, b=None):
         a = np.ascontiguousarray(a, dtype=np.float64)
