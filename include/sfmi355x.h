@@ -112,6 +112,23 @@ int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d
 int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
                          uint64_t *d_counts, void *stream);
 
+/* Plans: build once, execute many times.  A plan fixes the slices' shapes, bins, operator and
+ * precision, and the device pointers x and u of every slice; the data behind those pointers may
+ * change between executions (time slices updated in place).  sf_plan_execute enqueues the kernels
+ * on `stream` and accumulates into d_sums / d_counts; with use_graph != 0 the kernel sequence is
+ * captured into a hipGraph on the first execution for a given (outputs, stream) and replayed
+ * afterwards.  sf_plan_sync waits for the last execution; if the mixed kernel's re-bin queue
+ * overflowed (outputs then left untouched), it redoes that execution with the binary64 kernel and
+ * sets *redone.  One plan serves one execution at a time.
+ * (The reference re-stages and re-launches per call, KA ext :630-675; this is the persistent,
+ * graph-captured batched path.) */
+typedef struct sf_plan sf_plan;
+int sf_plan_create(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, sf_plan **out);
+int sf_plan_execute(sf_plan *plan, double *d_sums, uint64_t *d_counts, void *stream,
+                    int32_t use_graph);
+int sf_plan_sync(sf_plan *plan, int32_t *redone);
+void sf_plan_destroy(sf_plan *plan);
+
 /* Diagnostics of the last successful call on this context (for tests and benchmarks):
  * out[0] = pairs visited, out[1] = pairs re-binned in binary64 by the mixed kernel,
  * out[2] = work items, out[3] = workgroups launched, out[4] = histogram kernel time (ns),

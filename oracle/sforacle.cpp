@@ -264,7 +264,11 @@ int validate(const sfo_problem *p) {
     if (p->dist_edges[k] != p->dist_edges[k]) return -8;
     if (k > 0 && jl_isless(p->dist_edges[k], p->dist_edges[k - 1])) return -8;
   }
-  if (p->geometry == SFO_GEOM_FLAT && p->dist_edges[0] < 0.0) return -8;
+  // flat: squared plan needs non-negative edges; a leading -Inf is InfPaddedBinEdges' catch-all
+  // underflow bin (SquaredInfPaddedPlan, BinEdges.jl:434-437,520,553-558)
+  if (p->geometry == SFO_GEOM_FLAT)
+    for (int k = 0; k <= p->n_dist; ++k)
+      if (p->dist_edges[k] < 0.0 && !(k == 0 && p->dist_edges[0] == -__builtin_inf())) return -8;
   if (p->n_val < 0) return -9;
   if (p->n_val > 0) {
     if (!p->val_edges) return -9;
@@ -340,7 +344,10 @@ void prepare(Prob &P) {
     default: P.order = p->nl; break;
   }
   P.sqedges.resize(p->n_dist + 1);
-  for (int k = 0; k <= p->n_dist; ++k) P.sqedges[k] = p->dist_edges[k] * p->dist_edges[k];
+  for (int k = 0; k <= p->n_dist; ++k) {
+    const double e = p->dist_edges[k];
+    P.sqedges[k] = (e == -__builtin_inf()) ? e : e * e;  // -Inf stays below every r^2
+  }
   if (p->geometry != SFO_GEOM_FLAT) {
     P.P.resize(3 * p->N); P.E.resize(3 * p->N); P.Nn.resize(3 * p->N);
     for (int64_t i = 0; i < p->N; ++i)
@@ -424,7 +431,10 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
     *distance = sqrt(r2);
     *value = sf_raw(o, du, dx, r2);
     std::vector<double> sq(p->n_dist + 1);
-    for (int k = 0; k <= p->n_dist; ++k) sq[k] = p->dist_edges[k] * p->dist_edges[k];
+    for (int k = 0; k <= p->n_dist; ++k) {
+      const double e = p->dist_edges[k];
+      sq[k] = (e == -__builtin_inf()) ? e : e * e;
+    }
     *dbin = count_isless(sq.data(), p->n_dist + 1, r2);
     if (p->n_val > 0) *vbin = count_isless(p->val_edges, p->n_val + 1, *value);
     return 1;

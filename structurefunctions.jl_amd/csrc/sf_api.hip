@@ -30,6 +30,10 @@ hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Param
 hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st);
 hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, double *dst_s,
                         unsigned long long *dst_c, int64_t n, hipStream_t st);
+hipError_t launch_merge_checked(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                                unsigned long long *dst_c, int64_t n,
+                                const unsigned long long *pcount, int64_t cap, int *flag,
+                                hipStream_t st);
 hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
@@ -146,18 +150,18 @@ Partition make_partition(const std::vector<long double> &keys, int cap_cells, in
 
 }  // namespace
 
-struct sf_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int n_cu = 0;
-  size_t max_smem = 0;
-  std::string err;
-  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  bool ev_pending = false;
-  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
-  // device buffers
-  DevBuf<double> d_x, d_u;
+namespace {
+// A plan holds everything one batch shape needs on the device: per-point records, bin tables,
+// work items, accumulators and the re-bin queue.  Building it does the host work and the
+// uploads; launching it enqueues kernels only, so a launch can be captured into a hipGraph and
+// replayed while the caller rewrites x/u in place (time slices).
+struct Plan {
+  sf_ctx *ctx = nullptr;
+  int n = 0, G = 0, D = 0, geom_code = 0, ncell = 0, S64 = 0, F32 = 0;
+  bool J = false, mixed = false, sload = false;
+  int block = 256, tj = 256, copies = 1, grid = 1, ncell_max = 1, n_items = 0;
+  int64_t total_pts = 0, max_n = 0, total_pairs = 0, cap = 0;
+  Params P{};
   DevBuf<double> rec64;
   DevBuf<float> rec32;
   DevBuf<Slice> slices;
@@ -171,6 +175,35 @@ struct sf_ctx {
   DevBuf<unsigned long long> pcount;
   DevBuf<Pending> pending;
   DevBuf<PrepSliceHost> prep;
+  DevBuf<int> flag;
+  hipGraphExec_t graph = nullptr;
+  double *g_sums = nullptr;
+  unsigned long long *g_counts = nullptr;
+  hipStream_t g_stream = nullptr;
+  void release() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    rec64.release(); rec32.release(); slices.release(); items.release(); cells.release();
+    dedges.release(); vedges.release(); vedges32.release(); acc_s.release(); acc_c.release();
+    counter.release(); pcount.release(); pending.release(); prep.release(); flag.release();
+  }
+};
+
+}  // namespace
+
+struct sf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 0;
+  size_t max_smem = 0;
+  std::string err;
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool ev_pending = false;
+  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
+  // device buffers: host-input staging and the plan reused by one-shot calls
+  DevBuf<double> d_x, d_u;
+  Plan oneshot;
 
   int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -239,8 +272,12 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
     if (k > 0 && jl_isless(e, p->dist_edges[k - 1]))
       return ctx->fail(SF_E_ARG, "distance_bins must be sorted ascending");
   }
-  if (p->geometry == SF_GEOM_FLAT && p->dist_edges[0] < 0.0)
-    return ctx->fail(SF_E_ARG, "Euclidean distance_bins must be non-negative (squared-distance plan)");
+  if (p->geometry == SF_GEOM_FLAT)
+    for (int k = 0; k <= p->n_dist; ++k)
+      if (p->dist_edges[k] < 0.0 && !(k == 0 && p->dist_edges[0] == -INFINITY))
+        return ctx->fail(SF_E_ARG,
+                         "Euclidean distance_bins must be non-negative (squared-distance plan); only a "
+                         "leading -Inf (InfPaddedBinEdges catch-all) is allowed");
   if (p->n_val < 0) return ctx->fail(SF_E_ARG, "n_val must be >= 0");
   if (p->n_val > 0) {
     if (!p->val_edges) return ctx->fail(SF_E_ARG, "value_bins missing");
@@ -302,7 +339,9 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   std::vector<long double> k64(nE);
   for (int k = 0; k < nE; ++k) {
     const double e = p.dist_edges[k];
-    T.dedges[k] = flat ? e * e : (e == 0.0 ? 0.0 : e);  // -0.0 -> +0.0 (keys are > 0)
+    // flat: squared edges (a leading -Inf, InfPaddedBinEdges' underflow edge, stays -Inf);
+    // sphere: the edges themselves, -0.0 -> +0.0 (keys are > 0)
+    T.dedges[k] = flat ? (e == -INFINITY ? e : e * e) : (e == 0.0 ? 0.0 : e);
     k64[k] = T.dedges[k];
   }
   Partition P64 = make_partition(k64, 1024, 6);
@@ -328,7 +367,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   for (int k = 0; k < nE; ++k) {
     const long double e = p.dist_edges[k];
     if (flat) {
-      C[k] = e * e;
+      C[k] = (e == -INFINITY) ? e : e * e;
     } else if (e <= 0) {
       C[k] = e;
     } else {
@@ -360,7 +399,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
     const float Fn = (c + 1 >= n32) ? INFINITY : (float)P32.low(c + 1);
     int nint = 0, which = -1, below = 0;
     for (int k = 0; k < nE; ++k) {
-      if (hi[k] < Fc) { ++below; continue; }
+      if (C[k] == -INFINITY || hi[k] < Fc) { ++below; continue; }  // -Inf: below every key
       if (lo[k] >= Fn) continue;
       ++nint;
       which = k;
@@ -446,10 +485,9 @@ extern "C" void sf_destroy(sf_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  ctx->d_x.release(); ctx->d_u.release(); ctx->rec64.release(); ctx->rec32.release();
-  ctx->slices.release(); ctx->items.release(); ctx->cells.release(); ctx->dedges.release();
-  ctx->vedges.release(); ctx->vedges32.release(); ctx->acc_s.release(); ctx->acc_c.release();
-  ctx->counter.release(); ctx->pcount.release(); ctx->pending.release(); ctx->prep.release();
+  ctx->d_x.release();
+  ctx->d_u.release();
+  ctx->oneshot.release();
   for (int k = 0; k < 4; ++k)
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
@@ -524,45 +562,43 @@ extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_
 
 namespace {
 
-// Core: slices' x/u are device pointers (for the host API they were uploaded first).  Accumulates
-// into device outputs out_s/out_c (n_slices * ncell each).
-int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
-              const double *const *du, const double *xmax, double *out_s,
-              unsigned long long *out_c, hipStream_t st) {
+// Host work and uploads for one batch.  x/u of each slice are device pointers.
+int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double *const *dx,
+               const double *const *du, const double *xmax, hipStream_t st) {
   const sf_problem &p0 = sl[0];
+  pl.ctx = ctx;
+  pl.n = n;
   const bool flat = p0.geometry == SF_GEOM_FLAT;
-  const int G = flat ? kFlat : kSphere;
-  const int D = p0.D;
-  const bool J = p0.n_val > 0;
-  const bool mixed = p0.precision == SF_PREC_MIXED;
-  const int ncell = p0.n_dist * (J ? p0.n_val : 1);
-  const int S64 = rec_d64(G, D);
-  const int F32 = rec_f32(G, D);
+  pl.G = flat ? kFlat : kSphere;
+  pl.D = p0.D;
+  pl.geom_code = p0.geometry;
+  pl.J = p0.n_val > 0;
+  pl.mixed = p0.precision == SF_PREC_MIXED;
+  pl.ncell = p0.n_dist * (pl.J ? p0.n_val : 1);
+  pl.S64 = rec_d64(pl.G, pl.D);
+  pl.F32 = rec_f32(pl.G, pl.D);
+  const int G = pl.G, D = pl.D, ncell = pl.ncell, S64 = pl.S64, F32 = pl.F32;
+  const bool J = pl.J, mixed = pl.mixed;
 
   // ---- records
-  int64_t total_pts = 0, max_n = 0, total_pairs = 0;
+  pl.total_pts = 0;
+  pl.max_n = 0;
+  pl.total_pairs = 0;
   std::vector<PrepSliceHost> prep(n);
   for (int s = 0; s < n; ++s) {
     prep[s].x = dx[s];
     prep[s].u = du[s];
     prep[s].N = sl[s].N;
-    prep[s].pt_off = total_pts;
+    prep[s].pt_off = pl.total_pts;
     prep[s].xscale = 0;
-    total_pts += sl[s].N;
-    max_n = std::max<int64_t>(max_n, sl[s].N);
-    total_pairs += pairs_of(sl[s].N);
+    pl.total_pts += sl[s].N;
+    pl.max_n = std::max<int64_t>(pl.max_n, sl[s].N);
+    pl.total_pairs += pairs_of(sl[s].N);
   }
-  for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
-  ctx->stats[0] = total_pairs;
-  ctx->ev_pending = false;
-  if (total_pairs == 0) return SF_OK;
-  for (int k = 0; k < 4; ++k)
-    if (!ctx->ev[k]) HIPCHK(hipEventCreate(&ctx->ev[k]));
-  HIPCHK(ctx->rec64.ensure((size_t)total_pts * S64 + 16));
-  if (mixed) HIPCHK(ctx->rec32.ensure((size_t)total_pts * F32 + 16));
-  HIPCHK(ctx->prep.ensure(n));
-  HIPCHK(hipMemcpyAsync(ctx->prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(launch_prep(G, D, mixed, p0.geometry, ctx->prep.p, n, max_n, ctx->rec64.p, ctx->rec32.p, st));
+  HIPCHK(pl.rec64.ensure((size_t)pl.total_pts * S64 + 16));
+  if (mixed) HIPCHK(pl.rec32.ensure((size_t)pl.total_pts * F32 + 16));
+  HIPCHK(pl.prep.ensure(n));
+  HIPCHK(hipMemcpyAsync(pl.prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
 
   // ---- tables
   std::vector<Slice> sdesc(n);
@@ -594,11 +630,12 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
     sdesc[s] = d;
   }
   if (vedges.empty()) { vedges.push_back(0.0); vedges32.push_back(0.0f); }
+  pl.ncell_max = ncell_max;
 
   // ---- launch geometry: one histogram per workgroup; large joint histograms get 1024-thread
   // workgroups (one per CU, 16 waves share the table), small 1D histograms 256-thread workgroups
   // with per-lane interleaved copies (several workgroups per CU)
-  int block = ctx->tune_block ? ctx->tune_block : (J ? 1024 : 256);
+  const int block = ctx->tune_block ? ctx->tune_block : (J ? 1024 : 256);
   const int tj = std::min(block, 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
   // (binary32 kernel only); 0 picks the default
@@ -612,10 +649,11 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
       copies >>= 1;
   }
   const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies);
-  if ((size_t)L.total > ctx->max_smem)
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies);
+  if ((size_t)std::max(L.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
-                     ncell, L.total, ctx->max_smem);
+                     ncell, std::max(L.total, L64.total), ctx->max_smem);
   Params Pk{};
   Pk.op = p0.op;
   Pk.nl = p0.nl;
@@ -625,11 +663,11 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   if (ctx->tune_bpc) bpc = std::min(bpc, ctx->tune_bpc);
   const int max_blocks = bpc * ctx->n_cu;
 
-  // ---- work items: (slice, i-tile, j-chunk); slices in descending N, large chunks first
+  // ---- work items: (slice, i-tile, j-chunk); slices in descending N
   int jchunk = ctx->tune_jchunk;
   if (jchunk <= 0) {
     const double target_items = 16.0 * max_blocks;
-    double per_item = (double)total_pairs / target_items;  // pairs per item
+    double per_item = (double)pl.total_pairs / target_items;  // pairs per item
     int jc = (int)(per_item / block);
     jc = std::max(jc, block);
     jc = std::min(jc, 64 * block);
@@ -652,54 +690,60 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
       }
     }
   }
-  const int n_items = (int)items.size();
-  const int grid = std::max(1, std::min(n_items, max_blocks));
-  ctx->stats[2] = n_items;
-  ctx->stats[3] = grid;
-  ctx->stats[7] = block;
+  if (items.empty()) items.push_back(Item{0, 0, 0, 0});
+  pl.n_items = (int)items.size();
+  pl.grid = std::max(1, std::min(pl.n_items, max_blocks));
+  pl.block = block;
+  pl.tj = tj;
+  pl.copies = copies;
+  pl.sload = sload;
 
   // ---- upload
-  HIPCHK(ctx->slices.ensure(n));
-  HIPCHK(ctx->items.ensure(items.size()));
-  HIPCHK(ctx->cells.ensure(cells.size()));
-  HIPCHK(ctx->dedges.ensure(dedges.size()));
-  HIPCHK(ctx->vedges.ensure(vedges.size()));
-  HIPCHK(ctx->vedges32.ensure(vedges32.size()));
-  HIPCHK(ctx->acc_s.ensure((size_t)n * ncell));
-  HIPCHK(ctx->acc_c.ensure((size_t)n * ncell));
-  HIPCHK(ctx->counter.ensure(4));
-  HIPCHK(ctx->pcount.ensure(4));
-  HIPCHK(hipMemcpyAsync(ctx->slices.p, sdesc.data(), sizeof(Slice) * n, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctx->items.p, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctx->cells.p, cells.data(), sizeof(Cell) * cells.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctx->dedges.p, dedges.data(), sizeof(double) * dedges.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctx->vedges.p, vedges.data(), sizeof(double) * vedges.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(ctx->vedges32.p, vedges32.data(), sizeof(float) * vedges32.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(pl.slices.ensure(n));
+  HIPCHK(pl.items.ensure(items.size()));
+  HIPCHK(pl.cells.ensure(cells.size()));
+  HIPCHK(pl.dedges.ensure(dedges.size()));
+  HIPCHK(pl.vedges.ensure(vedges.size()));
+  HIPCHK(pl.vedges32.ensure(vedges32.size()));
+  HIPCHK(pl.acc_s.ensure((size_t)n * ncell));
+  HIPCHK(pl.acc_c.ensure((size_t)n * ncell));
+  HIPCHK(pl.counter.ensure(4));
+  HIPCHK(pl.pcount.ensure(4));
+  HIPCHK(pl.flag.ensure(4));
+  HIPCHK(hipMemcpyAsync(pl.slices.p, sdesc.data(), sizeof(Slice) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pl.items.p, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pl.cells.p, cells.data(), sizeof(Cell) * cells.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pl.dedges.p, dedges.data(), sizeof(double) * dedges.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pl.vedges.p, vedges.data(), sizeof(double) * vedges.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(pl.vedges32.p, vedges32.data(), sizeof(float) * vedges32.size(), hipMemcpyHostToDevice, st));
+  // host vectors go out of scope: make the asynchronous copies complete first
+  HIPCHK(hipStreamSynchronize(st));
 
-  int64_t cap = 0;
+  pl.cap = 0;
   if (mixed) {
-    cap = std::max<int64_t>(1LL << 22, total_pairs / 2000);
-    cap = std::min<int64_t>(cap, 1LL << 27);
-    HIPCHK(ctx->pending.ensure((size_t)cap));
+    pl.cap = std::max<int64_t>(1LL << 22, pl.total_pairs / 2000);
+    pl.cap = std::min<int64_t>(pl.cap, 1LL << 27);
+    HIPCHK(pl.pending.ensure((size_t)pl.cap));
   }
 
-  Params P{};
-  P.slices = ctx->slices.p;
-  P.items = ctx->items.p;
-  P.n_items = n_items;
+  Params &P = pl.P;
+  P = Params{};
+  P.slices = pl.slices.p;
+  P.items = pl.items.p;
+  P.n_items = pl.n_items;
   P.block_rows = block;
-  P.counter = ctx->counter.p;
-  P.rec64 = ctx->rec64.p;
-  P.rec32 = ctx->rec32.p;
-  P.cells = ctx->cells.p;
-  P.dedges = ctx->dedges.p;
-  P.vedges = ctx->vedges.p;
-  P.vedges32 = ctx->vedges32.p;
-  P.sums = ctx->acc_s.p;
-  P.counts = ctx->acc_c.p;
-  P.pending = ctx->pending.p;
-  P.pending_count = ctx->pcount.p;
-  P.pending_cap = cap;
+  P.counter = pl.counter.p;
+  P.rec64 = pl.rec64.p;
+  P.rec32 = pl.rec32.p;
+  P.cells = pl.cells.p;
+  P.dedges = pl.dedges.p;
+  P.vedges = pl.vedges.p;
+  P.vedges32 = pl.vedges32.p;
+  P.sums = pl.acc_s.p;
+  P.counts = pl.acc_c.p;
+  P.pending = pl.pending.p;
+  P.pending_count = pl.pcount.p;
+  P.pending_cap = pl.cap;
   P.stats = nullptr;
   P.ncell_max = ncell_max;
   P.tj = tj;
@@ -714,32 +758,70 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
   P.radius = (p0.geometry == SF_GEOM_SPHERE_DEG) ? p0.radius : 1.0;
   // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + c^2 / sin(sigma))
   P.vcoef = (float)ldexp(16.0 * std::max(1, P.order), -24);
+  return SF_OK;
+}
 
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    const bool use_mixed = mixed && attempt == 0;
-    HIPCHK(launch_zero(ctx->acc_s.p, sizeof(double) * (int64_t)n * ncell, st));
-    HIPCHK(launch_zero(ctx->acc_c.p, sizeof(unsigned long long) * (int64_t)n * ncell, st));
-    HIPCHK(launch_zero(ctx->counter.p, sizeof(int) * 4, st));
-    HIPCHK(launch_zero(ctx->pcount.p, sizeof(unsigned long long) * 4, st));
-    const bool use_sload = use_mixed && sload;
-    const int stage = use_sload ? 0 : tj * (use_mixed ? F32 * 4 : S64 * 8);
-    P.stage_bytes = stage;
-    const SmemLayout L2 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage, copies);
-    HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(launch_hist(G, D, J, use_mixed, use_sload, P, grid, block, (size_t)L2.total, st));
-    HIPCHK(hipEventRecord(ctx->ev[1], st));
-    if (use_mixed) HIPCHK(launch_recheck(G, D, J, P, ctx->n_cu, st));
-    HIPCHK(hipEventRecord(ctx->ev[2], st));
-    if (!use_mixed) break;
+// Kernel-only launch sequence: records from the current x/u, zeroed accumulators, histogram,
+// re-bin (mixed), merge into out (skipped, with *flag set, if the re-bin queue overflowed).
+// ev (optional) brackets the histogram and re-bin kernels for timing.
+int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out_c, hipStream_t st,
+                hipEvent_t *ev) {
+  sf_ctx *ctx = pl.ctx;
+  if (pl.total_pairs == 0) return SF_OK;
+  const int64_t ncells = (int64_t)pl.n * pl.ncell;
+  HIPCHK(launch_prep(pl.G, pl.D, use_mixed, pl.geom_code, pl.prep.p, pl.n, pl.max_n, pl.rec64.p,
+                     pl.rec32.p, st));
+  HIPCHK(launch_zero(pl.acc_s.p, sizeof(double) * ncells, st));
+  HIPCHK(launch_zero(pl.acc_c.p, sizeof(unsigned long long) * ncells, st));
+  HIPCHK(launch_zero(pl.counter.p, sizeof(int) * 4, st));
+  HIPCHK(launch_zero(pl.pcount.p, sizeof(unsigned long long) * 4, st));
+  HIPCHK(launch_zero(pl.flag.p, sizeof(int) * 4, st));
+  const bool use_sload = use_mixed && pl.sload;
+  const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
+  Params P = pl.P;
+  P.stage_bytes = stage;
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies);
+  if (ev) HIPCHK(hipEventRecord(ev[0], st));
+  HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
+  if (ev) HIPCHK(hipEventRecord(ev[1], st));
+  if (use_mixed) HIPCHK(launch_recheck(pl.G, pl.D, pl.J, P, ctx->n_cu, st));
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
+  HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
+                              use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  return SF_OK;
+}
+
+// One-shot: build (into the context's cached plan, reusing its buffers), launch, and on a
+// re-bin queue overflow redo the call with the binary64 kernel.
+int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
+              const double *const *du, const double *xmax, double *out_s,
+              unsigned long long *out_c, hipStream_t st) {
+  Plan &pl = ctx->oneshot;
+  int rc = plan_build(ctx, pl, n, sl, dx, du, xmax, st);
+  if (rc) return rc;
+  for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
+  ctx->stats[0] = pl.total_pairs;
+  ctx->stats[2] = pl.n_items;
+  ctx->stats[3] = pl.grid;
+  ctx->stats[7] = pl.block;
+  ctx->ev_pending = false;
+  if (pl.total_pairs == 0) return SF_OK;
+  for (int k = 0; k < 4; ++k)
+    if (!ctx->ev[k]) HIPCHK(hipEventCreate(&ctx->ev[k]));
+  rc = plan_launch(pl, pl.mixed, out_s, out_c, st, ctx->ev);
+  if (rc) return rc;
+  if (pl.mixed) {
     unsigned long long pc = 0;
-    HIPCHK(hipMemcpyAsync(&pc, ctx->pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&pc, pl.pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats[1] = (int64_t)pc;
-    if ((int64_t)pc <= cap) break;
-    // queue overflow: discard and redo with the binary64 kernel
+    if ((int64_t)pc > pl.cap) {
+      // the merge was skipped; redo everything with the binary64 kernel
+      rc = plan_launch(pl, false, out_s, out_c, st, ctx->ev);
+      if (rc) return rc;
+    }
   }
-  HIPCHK(launch_merge(ctx->acc_s.p, ctx->acc_c.p, out_s, out_c, (int64_t)n * ncell, st));
-  HIPCHK(hipEventRecord(ctx->ev[3], st));
   ctx->ev_pending = true;
   return SF_OK;
 }
@@ -847,4 +929,121 @@ extern "C" int sf_hist_batch_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
 extern "C" int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d_counts,
                               void *stream) {
   return sf_hist_batch_device(ctx, 1, p, d_sums, d_counts, stream);
+}
+
+// ----------------------------------------------------------------------------------------------
+// plans: build once, execute many times (optionally as a captured hipGraph)
+// ----------------------------------------------------------------------------------------------
+struct sf_plan {
+  sf_ctx *ctx = nullptr;
+  Plan pl;
+  hipStream_t last_stream = nullptr;
+  double *last_s = nullptr;
+  unsigned long long *last_c = nullptr;
+};
+
+extern "C" int sf_plan_create(sf_ctx *ctx, int32_t n, const sf_problem *sl, sf_plan **out) {
+  if (!ctx || !out) return SF_E_ARG;
+  *out = nullptr;
+  ctx->err.clear();
+  int rc = validate_batch(ctx, n, sl, false);
+  if (rc) return rc;
+  if (n == 0) return ctx->fail(SF_E_ARG, "a plan needs at least one slice");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+    if (sl[s].precision == SF_PREC_MIXED && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+      // coordinate magnitude for the binary32 margin; later contents must stay within 2x of it
+      std::vector<double> hx((size_t)sl[s].W * sl[s].N);
+      HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double m = 0;
+      for (double v : hx) m = std::max(m, fabs(v));
+      xmax[s] = 2.0 * m;
+    }
+  }
+  sf_plan *plan = new sf_plan();
+  plan->ctx = ctx;
+  rc = plan_build(ctx, plan->pl, n, sl, px.data(), pu.data(), xmax.data(), st);
+  if (rc) {
+    plan->pl.release();
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return SF_OK;
+}
+
+extern "C" int sf_plan_execute(sf_plan *plan, double *d_sums, uint64_t *d_counts, void *stream,
+                               int32_t use_graph) {
+  if (!plan) return SF_E_ARG;
+  sf_ctx *ctx = plan->ctx;
+  if (!d_sums || !d_counts) return ctx->fail(SF_E_ARG, "null sums or counts");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  Plan &pl = plan->pl;
+  unsigned long long *dc = (unsigned long long *)d_counts;
+  plan->last_stream = st;
+  plan->last_s = d_sums;
+  plan->last_c = dc;
+  if (!use_graph) return plan_launch(pl, pl.mixed, d_sums, dc, st, nullptr);
+  if (!pl.graph || pl.g_sums != d_sums || pl.g_counts != dc || pl.g_stream != st) {
+    if (pl.graph) {
+      HIPCHK(hipGraphExecDestroy(pl.graph));
+      pl.graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = plan_launch(pl, pl.mixed, d_sums, dc, st, nullptr);
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return ctx->fail(SF_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      pl.graph = nullptr;
+      return ctx->fail(SF_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    pl.g_sums = d_sums;
+    pl.g_counts = dc;
+    pl.g_stream = st;
+  }
+  HIPCHK(hipGraphLaunch(pl.graph, st));
+  return SF_OK;
+}
+
+extern "C" int sf_plan_sync(sf_plan *plan, int32_t *redone) {
+  if (!plan) return SF_E_ARG;
+  sf_ctx *ctx = plan->ctx;
+  if (redone) *redone = 0;
+  if (!plan->last_stream) return SF_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  Plan &pl = plan->pl;
+  HIPCHK(hipStreamSynchronize(plan->last_stream));
+  if (!pl.mixed || pl.total_pairs == 0) return SF_OK;
+  int flag = 0;
+  HIPCHK(hipMemcpy(&flag, pl.flag.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) {
+    // the last execute overflowed the re-bin queue and left the outputs untouched
+    const int rc = plan_launch(pl, false, plan->last_s, plan->last_c, plan->last_stream, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(plan->last_stream));
+    if (redone) *redone = 1;
+  }
+  return SF_OK;
+}
+
+extern "C" void sf_plan_destroy(sf_plan *plan) {
+  if (!plan) return;
+  (void)hipSetDevice(plan->ctx->device);
+  if (plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
+  plan->pl.release();
+  delete plan;
 }

@@ -900,6 +900,22 @@ __global__ void merge_out(const double *src_s, const unsigned long long *src_c, 
   }
 }
 
+// merge unless the re-bin queue overflowed (then the accumulators are incomplete: flag it and
+// leave the outputs untouched so the host can redo the call in binary64)
+__global__ void merge_checked(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                              unsigned long long *dst_c, int64_t n,
+                              const unsigned long long *pcount, int64_t cap, int *flag) {
+  if (pcount && *pcount > (unsigned long long)cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 1;
+    return;
+  }
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    dst_s[k] += src_s[k];
+    dst_c[k] += src_c[k];
+  }
+}
+
 __global__ void zero_u64(unsigned long long *p, int64_t n) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0ull;
@@ -1039,6 +1055,16 @@ hipError_t launch_merge(const double *src_s, const unsigned long long *src_c, do
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st) {
   eval_math<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(fn, n, a, b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_checked(const double *src_s, const unsigned long long *src_c, double *dst_s,
+                                unsigned long long *dst_c, int64_t n,
+                                const unsigned long long *pcount, int64_t cap, int *flag,
+                                hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  merge_checked<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src_s, src_c, dst_s, dst_c, n, pcount,
+                                                             cap, flag);
   return hipGetLastError();
 }
 

@@ -1,0 +1,183 @@
+# StructureFunctionsMI355XExt.jl -- backend shim plugging libsfmi355x.so into StructureFunctions.jl.
+#
+# Adds an execution backend `MI355XBackend` and the four methods of the reference's backend hook
+#   _dispatch_execution_backend(backend, shape::PointField{D}, sf, x, u, distance_bins, CT; ...)
+#   _dispatch_execution_backend(backend, shape::PointField{D}, sf, x, u, distance_bins, value_bins; ...)
+#   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins; ...)
+#   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins, value_bins; ...)
+# (src/Calculations/dispatch.jl:377-477, 510-594).  The public entry points validate shapes and
+# count width and apply `_finalize` around these calls (dispatch.jl:64-136, 346-373), exactly as
+# for the Serial/Threaded/GPU backends.
+#
+# MI355XBackend subtypes CB.AbstractExecutionBackend, not CB.AbstractGPUBackend, so the generic
+# GPU methods (which reach into `backend.backend` and KernelAbstractions) never match it.
+#
+# Status: written against the reference's interfaces; there is no Julia toolchain in the build
+# environment, so this file is reviewed, not executed.  The identical C ABI is exercised by the
+# Python harness and the pytest suite.
+module StructureFunctionsMI355XExt
+
+using ComputationalBackends: ComputationalBackends as CB
+using Distances: Distances as DI
+using StructureFunctions:
+    StructureFunctions as SF, Calculations as SFC, StructureFunctionObjects as SFO,
+    StructureFunctionTypes as SFT
+
+export MI355XBackend
+
+const libsfmi355x = get(ENV, "SFMI355X_LIB", joinpath(@__DIR__, "..", "libsfmi355x.so"))
+
+"""
+    MI355XBackend(; device = 0, precision = :fp64)
+
+Run the O(N^2) pair histogram on one AMD MI355X (gfx950).  `precision = :fp64` is the drop-in
+binary64 kernel (counts bit-identical to SerialBackend); `precision = :mixed` runs the pair
+arithmetic in binary32 with binary64 re-binning of pairs near bin edges (counts still
+bit-identical) and binary64 accumulation.
+"""
+struct MI355XBackend <: CB.AbstractExecutionBackend
+    device::Int
+    precision::Symbol
+end
+MI355XBackend(; device::Integer = 0, precision::Symbol = :fp64) = MI355XBackend(device, precision)
+
+# ---- C ABI mirror (include/sfmi355x.h) -----------------------------------------------------
+struct SfProblem
+    D::Int32
+    W::Int32
+    N::Int64
+    x::Ptr{Float64}
+    u::Ptr{Float64}
+    geometry::Int32
+    radius::Float64
+    op::Int32
+    nl::Int32
+    nt::Int32
+    n_dist::Int32
+    dist_edges::Ptr{Float64}
+    n_val::Int32
+    val_edges::Ptr{Float64}
+    precision::Int32
+    count_bits::Int32
+end
+
+const _CONTEXTS = Dict{Int, Ptr{Cvoid}}()
+
+function _context(dev::Int)
+    get!(_CONTEXTS, dev) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:sf_create, libsfmi355x), Cint, (Cint, Ref{Ptr{Cvoid}}), dev, h)
+        rc == 0 || error("sf_create(device=$dev) failed with code $rc")
+        h[]
+    end
+end
+
+function _check(ctx, rc)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:sf_last_error, libsfmi355x), Cstring, (Ptr{Cvoid},), ctx))
+    rc == -2 && throw(DimensionMismatch(msg))
+    rc in (-1, -3, -5) && throw(ArgumentError(msg))
+    error("libsfmi355x: " * msg)
+end
+
+# operator codes (SF_OP_*)
+_opcode(::SFT.ProjectedStructureFunctionType{NL, NT}) where {NL, NT} = (Int32(0), Int32(NL), Int32(NT))
+_opcode(::SFT.SecondOrderStructureFunctionType) = (Int32(1), Int32(0), Int32(0))
+_opcode(::SFT.ThirdOrderStructureFunctionType) = (Int32(2), Int32(0), Int32(0))
+_opcode(::SFT.TransverseComponentSecondOrderStructureFunctionType) = (Int32(3), Int32(0), Int32(0))
+_opcode(::SFT.LongitudinalTransverseComponentThirdOrderStructureFunctionType) = (Int32(4), Int32(0), Int32(0))
+_opcode(::SFT.FullVectorStructureFunctionType{NF}) where {NF} = (Int32(5), Int32(NF), Int32(0))
+_opcode(sf) = throw(ArgumentError("$(typeof(sf)) is not supported by MI355XBackend"))
+
+# geometry codes (SF_GEOM_*), from pair_geometry_for (HelperFunctions.jl:390-393)
+_geometry(::DI.Euclidean) = (Int32(0), 1.0)
+_geometry(m::DI.Haversine) = (Int32(1), Float64(m.radius))
+_geometry(::DI.SphericalAngle) = (Int32(2), 1.0)
+_geometry(m) = throw(ArgumentError("no pair geometry is defined for distance_metric=$(typeof(m))"))
+
+# explicit binary64 edge vectors for every bin specification (BinEdges.jl)
+_edges(b::SF.LogBinEdges) = SF.physical_edges_vector(b)
+_edges(b::SF.InfPaddedBinEdges) = vcat(-Inf, collect(Float64, b.edges), Inf)
+_edges(b::AbstractVector) = collect(Float64, b)
+
+_count_bits(::Type{CT}) where {CT} = sizeof(CT) <= 4 ? Int32(32) : Int32(64)
+_prec(b::MI355XBackend) = b.precision === :mixed ? Int32(1) : Int32(0)
+
+function _run!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
+               dbins, vbins, metric, CT)
+    ctx = _context(b.device)
+    xc = convert(Matrix{Float64}, x)
+    uc = convert(Matrix{Float64}, u)
+    de = _edges(dbins)
+    ve = vbins === nothing ? Float64[] : _edges(vbins)
+    op, nl, nt = _opcode(sf)
+    geom, radius = _geometry(metric)
+    GC.@preserve xc uc de ve sums counts begin
+        p = SfProblem(Int32(size(uc, 1)), Int32(size(xc, 1)), Int64(size(xc, 2)),
+                      pointer(xc), pointer(uc), geom, radius, op, nl, nt,
+                      Int32(length(de) - 1), pointer(de),
+                      Int32(vbins === nothing ? 0 : length(ve) - 1),
+                      vbins === nothing ? Ptr{Float64}(C_NULL) : pointer(ve),
+                      _prec(b), _count_bits(CT))
+        rc = ccall((:sf_hist, libsfmi355x), Cint,
+                   (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
+                   ctx, p, sums, counts)
+        _check(ctx, rc)
+    end
+    return nothing
+end
+
+# ---- allocating methods ----------------------------------------------------------------------
+function SFC._dispatch_execution_backend(
+    b::MI355XBackend, ::SFC.PointField{D}, sf::SFT.AbstractPairwiseStructureFunctionType,
+    x, u, distance_bins, count_eltype::Type{CT} = UInt32;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {D, CT}
+    nb = length(distance_bins) - 1
+    sums = zeros(Float64, nb)
+    counts = zeros(UInt64, nb)
+    _run!(b, sums, counts, sf, x, u, distance_bins, nothing, distance_metric, CT)
+    return SFO.StructureFunctionSumsAndCounts(sf, distance_bins, sums, CT.(counts))
+end
+
+function SFC._dispatch_execution_backend(
+    b::MI355XBackend, ::SFC.PointField{D}, sf::SFT.AbstractPairwiseStructureFunctionType,
+    x, u, distance_bins, value_bins::AbstractVector;
+    distance_metric::DI.PreMetric = DI.Euclidean(), count_eltype::Type{CT} = UInt32, kwargs...,
+) where {D, CT}
+    nd, nv = length(distance_bins) - 1, length(value_bins) - 1
+    sums = zeros(Float64, nd * nv)
+    counts = zeros(UInt64, nd * nv)
+    _run!(b, sums, counts, sf, x, u, distance_bins, value_bins, distance_metric, CT)
+    return SFO.StructureFunction2DSumsAndCounts(sf, distance_bins, value_bins,
+        reshape(sums, nd, nv), reshape(CT.(counts), nd, nv))
+end
+
+# ---- mutating methods: accumulate, like calculate_structure_function! -------------------------
+function SFC._dispatch_execution_backend!(
+    b::MI355XBackend, ::SFC.PointField{D}, sums::AbstractVector, counts::AbstractVector,
+    sf::SFT.AbstractPairwiseStructureFunctionType, x, u, distance_bins;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {D}
+    s = zeros(Float64, length(sums))
+    c = zeros(UInt64, length(counts))
+    _run!(b, s, c, sf, x, u, distance_bins, nothing, distance_metric, eltype(counts))
+    sums .+= s
+    counts .+= c
+    return nothing
+end
+
+function SFC._dispatch_execution_backend!(
+    b::MI355XBackend, ::SFC.PointField{D}, sums::AbstractMatrix, counts::AbstractMatrix,
+    sf::SFT.AbstractPairwiseStructureFunctionType, x, u, distance_bins, value_bins::AbstractVector;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {D}
+    s = zeros(Float64, length(sums))
+    c = zeros(UInt64, length(counts))
+    _run!(b, s, c, sf, x, u, distance_bins, value_bins, distance_metric, eltype(counts))
+    sums .+= reshape(s, size(sums))
+    counts .+= reshape(c, size(counts))
+    return nothing
+end
+
+end # module

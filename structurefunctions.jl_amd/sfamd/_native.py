@@ -19,7 +19,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_check", "sf_eval_math",
-            "sf_set_tuning")
+            "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy")
 
 
 class SfProblem(ctypes.Structure):
@@ -68,6 +68,14 @@ def lib():
     L.sf_eval_math.restype = ctypes.c_int
     L.sf_set_tuning.argtypes = [vp, i32, i32, i32, i32]
     L.sf_set_tuning.restype = ctypes.c_int
+    L.sf_plan_create.argtypes = [vp, i32, ctypes.POINTER(SfProblem), ctypes.POINTER(vp)]
+    L.sf_plan_create.restype = ctypes.c_int
+    L.sf_plan_execute.argtypes = [vp, dp, dp, vp, i32]
+    L.sf_plan_execute.restype = ctypes.c_int
+    L.sf_plan_sync.argtypes = [vp, ctypes.POINTER(i32)]
+    L.sf_plan_sync.restype = ctypes.c_int
+    L.sf_plan_destroy.argtypes = [vp]
+    L.sf_plan_destroy.restype = None
     _lib = L
     return L
 
@@ -149,6 +157,37 @@ class Context:
                                         ctypes.c_void_p(d_counts_ptr),
                                         ctypes.c_void_p(stream_ptr) if stream_ptr else None)
         self.check(rc)
+
+
+class Plan:
+    """sf_plan: a batch built once and executed many times (optionally as a hipGraph)."""
+
+    def __init__(self, ctx: Context, problems):
+        self.ctx = ctx
+        self._problems = (SfProblem * len(problems))(*problems)
+        h = ctypes.c_void_p()
+        ctx.check(lib().sf_plan_create(ctx.handle, len(problems), self._problems, ctypes.byref(h)))
+        self._h = h
+
+    def execute(self, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0, graph: bool = True):
+        self.ctx.check(lib().sf_plan_execute(self._h, ctypes.c_void_p(d_sums_ptr), ctypes.c_void_p(d_counts_ptr),
+                                             ctypes.c_void_p(stream_ptr) if stream_ptr else None, 1 if graph else 0))
+
+    def sync(self) -> bool:
+        r = ctypes.c_int32(0)
+        self.ctx.check(lib().sf_plan_sync(self._h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sf_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def check_problems(problems) -> tuple[int, str]:
