@@ -112,6 +112,14 @@ int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d
 int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
                          uint64_t *d_counts, void *stream);
 
+/* Autotuner: times candidate launch shapes (threads per workgroup, column-range length, column
+ * variant) on these slices (device pointers, as for sf_hist_batch_device; outputs are discarded),
+ * keeps the fastest in the context keyed by (geometry, D, joint, precision, operator class,
+ * log2 of the pair count, n_dist, n_val), and writes it to out (block, j_chunk, blocks_per_cu,
+ * variant).  Later calls with a matching key and no sf_set_tuning override use it.  (No reference
+ * counterpart.) */
+int sf_autotune(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t out[4]);
+
 /* Plans: build once, execute many times.  A plan fixes the slices' shapes, bins, operator and
  * precision, and the device pointers x and u of every slice; the data behind those pointers may
  * change between executions (time slices updated in place).  sf_plan_execute enqueues the kernels

@@ -452,3 +452,112 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
   if (p->n_val > 0) *vbin = count_isless(p->val_edges, p->n_val + 1, *value);
   return f.ok ? 1 : 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Single pass: the six invariants S2, L2, T2, S3, L3, L1T2 in one pair pass
+// (calculate_structure_functions_single_pass, src/Calculations/serial_single_pass.jl:602-700).
+// Flat D in {2,3}: _pf_sp_simd_pairs! (:231-271): inv_r = inv(sqrt(r2)), du_L = dot(du,dx)*inv_r,
+// |du|^2 = dot(du,du); sphere: _sp1d_pairs! (:315-345): du_L = dot(du, rhat), |du|^2 = dot(du,du).
+// Rows 1,2,4,5 (S2, L2, S3, L3) are accumulated; T2 = S2 - L2 and L1T2 = S3 - L3 are derived per
+// bin and the count is shared (_sp1d_derive_rows!, :208-219).  Output layout (6, n_dist)
+// column-major, accumulated.
+// ---------------------------------------------------------------------------------------------
+extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t i_begin,
+                                    int64_t i_end, int64_t i_step, double *sums,
+                                    uint64_t *counts) {
+  sfo_problem q = *p;
+  q.op = SFO_OP_S2;
+  q.nl = q.nt = 0;
+  q.n_val = 0;
+  q.val_edges = nullptr;
+  const int rc = validate(&q);
+  if (rc) return rc;
+  if (!sums || !counts || i_step < 1) return -10;
+  if (i_begin < 0) i_begin = 0;
+  if (i_end > q.N) i_end = q.N;
+  Prob P;
+  P.p = &q;
+  prepare(P);
+  const int nb = q.n_dist, D = q.D, W = q.W;
+  if (nthreads < 1) nthreads = 1;
+  // per thread: 4 moments (compensated) + count per bin
+  struct SP { std::vector<double> s, c; std::vector<uint64_t> n; };
+  std::vector<SP> acc(nthreads);
+  for (auto &a : acc) { a.s.assign(4 * nb, 0.0); a.c.assign(4 * nb, 0.0); a.n.assign(nb, 0); }
+  auto add = [](SP &a, int b, int m, double v) {
+    double &s = a.s[4 * b + m], &c = a.c[4 * b + m];
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
+    s = t;
+  };
+  auto work = [&](int t) {
+    SP &a = acc[t];
+    int64_t k = 0;
+    for (int64_t i = i_begin; i < i_end; i += i_step, ++k) {
+      if ((int)(k % nthreads) != t) continue;
+      for (int64_t j = i + 1; j < q.N; ++j) {
+        double duL, dn2;
+        int64_t b;
+        if (q.geometry == SFO_GEOM_FLAT) {
+          double dx[3] = {0, 0, 0}, du[3] = {0, 0, 0};
+          for (int d = 0; d < D; ++d) {
+            dx[d] = q.x[d + W * j] - q.x[d + W * i];
+            du[d] = q.u[d + D * j] - q.u[d + D * i];
+          }
+          const double r2 = sdot(dx, dx, D);
+          const double inv_r = 1.0 / sqrt(r2);
+          duL = sdot(du, dx, D) * inv_r;
+          dn2 = sdot(du, du, D);
+          b = count_isless(P.sqedges.data(), nb + 1, r2);
+          if (!(1 <= b && b <= nb)) continue;
+        } else {
+          const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],
+                                            &P.E[3 * j], &P.Nn[3 * j], q.radius);
+          b = count_isless(q.dist_edges, nb + 1, f.distance);
+          if (!(f.ok && 1 <= b && b <= nb)) continue;
+          double du[3];
+          sphere_delta(f, &q.u[D * i], &q.u[D * j], D, du);
+          const double rh[3] = {1.0, 0.0, 0.0};
+          duL = sdot(du, rh, D);
+          dn2 = sdot(du, du, D);
+        }
+        const double duL2 = duL * duL;
+        add(a, (int)(b - 1), 0, dn2);
+        add(a, (int)(b - 1), 1, duL2);
+        add(a, (int)(b - 1), 2, duL * dn2);
+        add(a, (int)(b - 1), 3, duL * duL2);
+        a.n[b - 1] += 1;
+      }
+    }
+  };
+  if (nthreads == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(work, t);
+    for (auto &x : th) x.join();
+  }
+  for (int b = 0; b < nb; ++b) {
+    double m[4];
+    uint64_t n = 0;
+    for (int mm = 0; mm < 4; ++mm) {
+      double s = 0.0, c = 0.0;
+      for (int t = 0; t < nthreads; ++t) {
+        const double parts[2] = {acc[t].s[4 * b + mm], acc[t].c[4 * b + mm]};
+        for (double v : parts) {
+          const double tt = s + v;
+          if (fabs(s) >= fabs(v)) c += (s - tt) + v; else c += (v - tt) + s;
+          s = tt;
+        }
+      }
+      m[mm] = s + c;
+    }
+    for (int t = 0; t < nthreads; ++t) n += acc[t].n[b];
+    const double rows[6] = {m[0], m[1], m[0] - m[1], m[2], m[3], m[2] - m[3]};
+    for (int r = 0; r < 6; ++r) {
+      sums[r + 6 * b] += rows[r];
+      counts[r + 6 * b] += n;
+    }
+  }
+  return 0;
+}

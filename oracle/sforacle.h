@@ -55,6 +55,13 @@ int sfo_hist(const sfo_problem *p, int nthreads, int compensated, int64_t i_begi
              int64_t i_step, double *sums, uint64_t *counts, double *abs_sums,
              double *mag_sums);
 
+/* Single pass over the pairs accumulating the six invariants S2, L2, T2, S3, L3, L1T2
+ * (calculate_structure_functions_single_pass, src/Calculations/serial_single_pass.jl:602-700).
+ * p->op / value bins are ignored.  sums and counts are (6, n_dist) column-major, accumulated; the
+ * count row is shared by all six invariants; T2 = S2 - L2 and L1T2 = S3 - L3 per bin. */
+int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t i_begin, int64_t i_end,
+                         int64_t i_step, double *sums, uint64_t *counts);
+
 /* deterministic elementary functions (structurefunctions.jl_amd/csrc/sfmath.h), for tests */
 void sfo_sincosd(double x, double *s, double *c);
 void sfo_sincos(double x, double *s, double *c);

@@ -15,7 +15,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/sfmi355x.h"
@@ -201,6 +203,9 @@ struct sf_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool ev_pending = false;
   int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
+  // autotuner cache: (geometry, D, joint, mixed, operator class, log2 pairs, n_dist, n_val) ->
+  // (block, j_chunk, blocks_per_cu, variant)
+  std::map<std::tuple<int, int, int, int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;
   // device buffers: host-input staging and the plan reused by one-shot calls
   DevBuf<double> d_x, d_u;
   Plan oneshot;
@@ -635,11 +640,25 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   // ---- launch geometry: one histogram per workgroup; large joint histograms get 1024-thread
   // workgroups (one per CU, 16 waves share the table), small 1D histograms 256-thread workgroups
   // with per-lane interleaved copies (several workgroups per CU)
-  const int block = ctx->tune_block ? ctx->tune_block : (J ? 1024 : 256);
+  // launch shape: explicit tuning (sf_set_tuning) > autotuned entry for this key > defaults
+  Params Pk{};
+  Pk.op = p0.op;
+  Pk.nl = p0.nl;
+  Pk.nt = p0.nt;
+  int t_block = ctx->tune_block, t_jchunk = ctx->tune_jchunk, t_bpc = ctx->tune_bpc;
+  int t_variant = ctx->tune_variant;
+  if (!t_block && !t_jchunk && !t_bpc && !t_variant) {
+    int lg = 0;
+    for (int64_t v = pl.total_pairs; v > 1; v >>= 1) ++lg;
+    const auto key = std::make_tuple(G, D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val);
+    const auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) std::tie(t_block, t_jchunk, t_bpc, t_variant) = it->second;
+  }
+  const int block = t_block ? t_block : (J ? 1024 : 256);
   const int tj = std::min(block, 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
   // (binary32 kernel only); 0 picks the default
-  const bool sload = mixed && (ctx->tune_variant == 2 || ctx->tune_variant == 0);
+  const bool sload = mixed && (t_variant == 2 || t_variant == 0);
   const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
   const int stage_bytes = sload ? 0 : tj * rec_bytes;
   int copies = 1;
@@ -654,17 +673,13 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
                      ncell, std::max(L.total, L64.total), ctx->max_smem);
-  Params Pk{};
-  Pk.op = p0.op;
-  Pk.nl = p0.nl;
-  Pk.nt = p0.nt;
   int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, block, (size_t)L.total);
   if (bpc < 1) bpc = 1;
-  if (ctx->tune_bpc) bpc = std::min(bpc, ctx->tune_bpc);
+  if (t_bpc) bpc = std::min(bpc, t_bpc);
   const int max_blocks = bpc * ctx->n_cu;
 
   // ---- work items: (slice, i-tile, j-chunk); slices in descending N
-  int jchunk = ctx->tune_jchunk;
+  int jchunk = t_jchunk;
   if (jchunk <= 0) {
     const double target_items = 16.0 * max_blocks;
     double per_item = (double)pl.total_pairs / target_items;  // pairs per item
@@ -1046,4 +1061,76 @@ extern "C" void sf_plan_destroy(sf_plan *plan) {
   if (plan->last_stream) (void)hipStreamSynchronize(plan->last_stream);
   plan->pl.release();
   delete plan;
+}
+
+// ----------------------------------------------------------------------------------------------
+// autotuner
+// ----------------------------------------------------------------------------------------------
+extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t out[4]) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  int rc = validate_batch(ctx, n, sl, false);
+  if (rc) return rc;
+  if (n == 0) return ctx->fail(SF_E_ARG, "nothing to tune");
+  HIPCHK(hipSetDevice(ctx->device));
+  const sf_problem &p0 = sl[0];
+  const bool J = p0.n_val > 0, mixed = p0.precision == SF_PREC_MIXED;
+  const int ncell = p0.n_dist * (J ? p0.n_val : 1);
+  DevBuf<double> ts;
+  DevBuf<unsigned long long> tc;
+  HIPCHK(ts.ensure((size_t)n * ncell));
+  HIPCHK(tc.ensure((size_t)n * ncell));
+  std::vector<std::tuple<int, int, int, int>> cand;
+  const int blocks[3] = {256, 512, 1024};
+  for (int b : blocks) {
+    if (J && b < 512) continue;
+    if (!J && b > 512) continue;
+    for (int v = (mixed ? 1 : 0); v <= (mixed ? 2 : 0); ++v) {
+      for (int jc_scale = 0; jc_scale < 2; ++jc_scale) cand.emplace_back(b, jc_scale ? 4 * b : 0, 0, v);
+    }
+  }
+  const int saved[4] = {ctx->tune_block, ctx->tune_jchunk, ctx->tune_bpc, ctx->tune_variant};
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+    xmax[s] = 1.0;  // timing only: margins do not change the work, results are discarded
+  }
+  double best = 1e300;
+  std::tuple<int, int, int, int> best_c = cand.front();
+  for (const auto &c : cand) {
+    std::tie(ctx->tune_block, ctx->tune_jchunk, ctx->tune_bpc, ctx->tune_variant) = c;
+    double t = 0;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {
+      rc = run_batch(ctx, n, sl, px.data(), pu.data(), xmax.data(), ts.p, tc.p, ctx->stream);
+      int64_t st[8];
+      sf_last_stats(ctx, st);
+      t = (double)st[6];
+    }
+    if (rc) break;
+    if (t < best) {
+      best = t;
+      best_c = c;
+    }
+  }
+  ctx->tune_block = saved[0];
+  ctx->tune_jchunk = saved[1];
+  ctx->tune_bpc = saved[2];
+  ctx->tune_variant = saved[3];
+  ts.release();
+  tc.release();
+  if (rc) return rc;
+  int64_t total = 0;
+  for (int s = 0; s < n; ++s) total += pairs_of(sl[s].N);
+  int lg = 0;
+  for (int64_t v = total; v > 1; v >>= 1) ++lg;
+  Params Pk{};
+  Pk.op = p0.op;
+  Pk.nl = p0.nl;
+  Pk.nt = p0.nt;
+  const int G = p0.geometry == SF_GEOM_FLAT ? kFlat : kSphere;
+  ctx->tuned[std::make_tuple(G, p0.D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val)] = best_c;
+  if (out) std::tie(out[0], out[1], out[2], out[3]) = best_c;
+  return SF_OK;
 }

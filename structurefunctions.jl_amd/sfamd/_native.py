@@ -19,7 +19,8 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_check", "sf_eval_math",
-            "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy")
+            "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy",
+            "sf_autotune")
 
 
 class SfProblem(ctypes.Structure):
@@ -76,6 +77,8 @@ def lib():
     L.sf_plan_sync.restype = ctypes.c_int
     L.sf_plan_destroy.argtypes = [vp]
     L.sf_plan_destroy.restype = None
+    L.sf_autotune.argtypes = [vp, i32, ctypes.POINTER(SfProblem), ctypes.POINTER(i32)]
+    L.sf_autotune.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -133,6 +136,13 @@ class Context:
 
     def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0, variant=0):
         self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu), int(variant)))
+
+    def autotune(self, problems):
+        """sf_autotune on device-pointer problems; returns (block, j_chunk, blocks_per_cu, variant)."""
+        arr = (SfProblem * len(problems))(*problems)
+        out = (ctypes.c_int32 * 4)()
+        self.check(lib().sf_autotune(self._h, len(problems), arr, out))
+        return tuple(out)
 
     def eval_math(self, fn: int, a, b=None):
         a = np.ascontiguousarray(a, dtype=np.float64)

@@ -36,6 +36,10 @@ def lib():
                                ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                ctypes.c_void_p, ctypes.c_void_p]
         L.sfo_hist.restype = ctypes.c_int
+        L.sfo_hist_single_pass.argtypes = [ctypes.POINTER(Prob), ctypes.c_int, ctypes.c_int64,
+                                           ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                           ctypes.c_void_p]
+        L.sfo_hist_single_pass.restype = ctypes.c_int
         L.sfo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
         L.sfo_atan2.restype = ctypes.c_double
         L.sfo_sincosd.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -91,6 +95,19 @@ def hist(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, nt
     if nv:
         out = [v.reshape((nd, nv), order="F") for v in out]
     return tuple(out) if with_mag else tuple(out[:3])
+
+
+def single_pass(x, u, edges, geometry="flat", radius=1.0, nthreads=1, rows=None):
+    """Six invariants in one pass: (sums, counts), each (6, n_dist) rows S2, L2, T2, S3, L3, L1T2."""
+    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None)
+    nd = p.n_dist
+    s = np.zeros(6 * nd)
+    c = np.zeros(6 * nd, dtype=np.uint64)
+    i0, i1, st = (0, p.N, 1) if rows is None else rows
+    rc = lib().sfo_hist_single_pass(ctypes.byref(p), nthreads, i0, i1, st, s.ctypes.data, c.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the problem (code {rc})")
+    return s.reshape((6, nd), order="F"), c.reshape((6, nd), order="F")
 
 
 def pair(x, u, edges, i, j, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):

@@ -47,3 +47,19 @@ def test_oracle_joint_vs_mpmath(geometry, D):
     s, c, a = _oracle.hist(x, u, edges, op=(0, 1, 0), geometry=geometry, vedges=vedges)
     assert np.array_equal(c, mc)
     assert np.all(np.abs(s - ms) <= 1e-13 * np.maximum(ma, 1e-300) + 1e-300)
+
+
+@pytest.mark.parametrize("geometry,D", [("flat", 2), ("flat", 3), ("deg", 2), ("deg", 3)])
+def test_single_pass_vs_mpmath_and_single_operators(geometry, D):
+    """Single-pass invariants (serial_single_pass.jl:602-700): each row equals the brute force of
+    its operator, the count row is shared, and S2 = L2 + T2, S3 = L3 + L1T2 per bin."""
+    x, u, edges = case(geometry, D, 14, seed=300 + D)
+    s, c = _oracle.single_pass(x, u, edges, geometry=geometry)
+    rows = [(1, 0, 0), (0, 2, 0), (0, 0, 2), (2, 0, 0), (0, 3, 0), (0, 1, 2)]
+    for r, op in enumerate(rows):
+        ms, mc, ma, margin = mp_reference.brute(x, u, edges, op, geometry)
+        assert margin > 1e-9
+        assert np.array_equal(c[r], mc)
+        assert np.all(np.abs(s[r] - ms) <= 1e-12 * np.maximum(ma, 1e-300) + 1e-300), r
+    np.testing.assert_array_equal(s[2], s[0] - s[1])
+    np.testing.assert_array_equal(s[5], s[3] - s[4])
