@@ -458,9 +458,9 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
 // (calculate_structure_functions_single_pass, src/Calculations/serial_single_pass.jl:602-700).
 // Flat D in {2,3}: _pf_sp_simd_pairs! (:231-271): inv_r = inv(sqrt(r2)), du_L = dot(du,dx)*inv_r,
 // |du|^2 = dot(du,du); sphere: _sp1d_pairs! (:315-345): du_L = dot(du, rhat), |du|^2 = dot(du,du).
-// Rows 1,2,4,5 (S2, L2, S3, L3) are accumulated; T2 = S2 - L2 and L1T2 = S3 - L3 are derived per
-// bin and the count is shared (_sp1d_derive_rows!, :208-219).  Output layout (6, n_dist)
-// column-major, accumulated.
+// Rows 1,2,4,5 (S2, L2, S3, L3) and count row 1 are accumulated; then, over the whole buffer,
+// T2 = S2 - L2, L1T2 = S3 - L3 and count rows 2..6 = count row 1 (_sp1d_derive_rows!, :208-219).
+// Output layout (6, n_dist) column-major.
 // ---------------------------------------------------------------------------------------------
 extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t i_begin,
                                     int64_t i_end, int64_t i_step, double *sums,
@@ -553,11 +553,17 @@ extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t 
       m[mm] = s + c;
     }
     for (int t = 0; t < nthreads; ++t) n += acc[t].n[b];
-    const double rows[6] = {m[0], m[1], m[0] - m[1], m[2], m[3], m[2] - m[3]};
-    for (int r = 0; r < 6; ++r) {
-      sums[r + 6 * b] += rows[r];
-      counts[r + 6 * b] += n;
-    }
+    sums[0 + 6 * b] += m[0];
+    sums[1 + 6 * b] += m[1];
+    sums[3 + 6 * b] += m[2];
+    sums[4 + 6 * b] += m[3];
+    counts[6 * b] += n;
+  }
+  // _sp1d_derive_rows! (serial_single_pass.jl:208-219): assignments over the whole buffer
+  for (int b = 0; b < nb; ++b) {
+    sums[2 + 6 * b] = sums[0 + 6 * b] - sums[1 + 6 * b];
+    sums[5 + 6 * b] = sums[3 + 6 * b] - sums[4 + 6 * b];
+    for (int r = 1; r < 6; ++r) counts[r + 6 * b] = counts[6 * b];
   }
   return 0;
 }

@@ -57,8 +57,9 @@ int sfo_hist(const sfo_problem *p, int nthreads, int compensated, int64_t i_begi
 
 /* Single pass over the pairs accumulating the six invariants S2, L2, T2, S3, L3, L1T2
  * (calculate_structure_functions_single_pass, src/Calculations/serial_single_pass.jl:602-700).
- * p->op / value bins are ignored.  sums and counts are (6, n_dist) column-major, accumulated; the
- * count row is shared by all six invariants; T2 = S2 - L2 and L1T2 = S3 - L3 per bin. */
+ * p->op / value bins are ignored.  sums and counts are (6, n_dist) column-major; rows S2, L2, S3,
+ * L3 and count row 1 accumulate, then T2 = S2 - L2, L1T2 = S3 - L3 and count rows 2..6 = count
+ * row 1 are assigned over the whole buffer (_sp1d_derive_rows!, serial_single_pass.jl:208-219). */
 int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t i_begin, int64_t i_end,
                          int64_t i_step, double *sums, uint64_t *counts);
 

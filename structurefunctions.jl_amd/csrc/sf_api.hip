@@ -37,6 +37,7 @@ hipError_t launch_merge_checked(const double *src_s, const unsigned long long *s
                                 const unsigned long long *pcount, int64_t cap, int *flag,
                                 hipStream_t st);
 hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
+hipError_t launch_derive_sp(double *s, unsigned long long *c, int64_t nbins, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
 int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, int block, size_t smem);
@@ -91,6 +92,7 @@ int op_order(const sf_problem &p) {
     case SF_OP_S3: return 3;
     case SF_OP_T2COMP: return 2;
     case SF_OP_L1T2COMP: return 3;
+    case SF_OP_SINGLE_PASS: return 3;
     default: return p.nl;
   }
 }
@@ -160,7 +162,7 @@ namespace {
 struct Plan {
   sf_ctx *ctx = nullptr;
   int n = 0, G = 0, D = 0, geom_code = 0, ncell = 0, S64 = 0, F32 = 0;
-  bool J = false, mixed = false, sload = false;
+  bool J = false, mixed = false, sload = false, sp = false;
   int block = 256, tj = 256, copies = 1, grid = 1, ncell_max = 1, n_items = 0;
   int64_t total_pts = 0, max_n = 0, total_pairs = 0, cap = 0;
   Params P{};
@@ -265,6 +267,11 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
     case SF_OP_FULLVEC:
       if (p->nl < 1 || p->nl > 3)
         return ctx->fail(SF_E_UNSUPPORTED, "FullVectorStructureFunctionType{%d}: NF must be 1..3", p->nl);
+      break;
+    case SF_OP_SINGLE_PASS:
+      if (p->n_val != 0)
+        return ctx->fail(SF_E_UNSUPPORTED,
+                         "the single-pass invariants are binned by distance only here (n_val must be 0)");
       break;
     default:
       return ctx->fail(SF_E_ARG, "unknown operator code %d (derived operators are not pairwise)", p->op);
@@ -460,6 +467,23 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
 
 int64_t pairs_of(int64_t n) { return n > 1 ? n * (n - 1) / 2 : 0; }
 
+// output cells per slice: (n_dist), (n_dist, n_val), or (6, n_dist) for the single pass
+int out_cells(const sf_problem &p) {
+  if (p.op == SF_OP_SINGLE_PASS) return kSpRows * p.n_dist;
+  return p.n_dist * (p.n_val > 0 ? p.n_val : 1);
+}
+
+// _sp1d_derive_rows! (serial_single_pass.jl:208-219) on host memory
+void derive_sp_host(double *s, uint64_t *c, int64_t nbins) {
+  for (int64_t b = 0; b < nbins; ++b) {
+    double *col = s + kSpRows * b;
+    uint64_t *cc = c + kSpRows * b;
+    col[2] = col[0] - col[1];
+    col[5] = col[3] - col[4];
+    for (int t = 1; t < kSpRows; ++t) cc[t] = cc[0];
+  }
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------------------------------------
@@ -579,7 +603,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   pl.geom_code = p0.geometry;
   pl.J = p0.n_val > 0;
   pl.mixed = p0.precision == SF_PREC_MIXED;
-  pl.ncell = p0.n_dist * (pl.J ? p0.n_val : 1);
+  pl.ncell = out_cells(p0);
+  pl.sp = p0.op == SF_OP_SINGLE_PASS;
   pl.S64 = rec_d64(pl.G, pl.D);
   pl.F32 = rec_f32(pl.G, pl.D);
   const int G = pl.G, D = pl.D, ncell = pl.ncell, S64 = pl.S64, F32 = pl.F32;
@@ -645,6 +670,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   Pk.op = p0.op;
   Pk.nl = p0.nl;
   Pk.nt = p0.nt;
+  Pk.nmom = pl.sp ? 4 : 1;
+  const int nmom = Pk.nmom;
   int t_block = ctx->tune_block, t_jchunk = ctx->tune_jchunk, t_bpc = ctx->tune_bpc;
   int t_variant = ctx->tune_variant;
   if (!t_block && !t_jchunk && !t_bpc && !t_variant) {
@@ -664,11 +691,11 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   int copies = 1;
   if (!J) {
     copies = 64;
-    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies).total * 2 > ctx->max_smem)
+    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom).total * 2 > ctx->max_smem)
       copies >>= 1;
   }
-  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies);
-  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies);
+  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom);
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom);
   if ((size_t)std::max(L.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
@@ -773,6 +800,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   P.radius = (p0.geometry == SF_GEOM_SPHERE_DEG) ? p0.radius : 1.0;
   // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + c^2 / sin(sigma))
   P.vcoef = (float)ldexp(16.0 * std::max(1, P.order), -24);
+  P.nmom = nmom;
   return SF_OK;
 }
 
@@ -795,7 +823,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
   Params P = pl.P;
   P.stage_bytes = stage;
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
@@ -803,6 +831,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
   HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
                               use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
+  if (pl.sp) HIPCHK(launch_derive_sp(out_s, out_c, (int64_t)pl.n * pl.P.n_dist, st));
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   return SF_OK;
 }
@@ -850,7 +879,7 @@ int host_entry(sf_ctx *ctx, int n, const sf_problem *sl, double *sums, uint64_t 
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const sf_problem &p0 = sl[0];
-  const int ncell = p0.n_dist * (p0.n_val > 0 ? p0.n_val : 1);
+  const int ncell = out_cells(p0);
   int64_t tx = 0, tu = 0;
   for (int s = 0; s < n; ++s) {
     tx += (int64_t)sl[s].W * sl[s].N;
@@ -896,6 +925,7 @@ int host_entry(sf_ctx *ctx, int n, const sf_problem *sl, double *sums, uint64_t 
     sums[k] += hs[k];
     counts[k] += hc[k];
   }
+  if (p0.op == SF_OP_SINGLE_PASS) derive_sp_host(sums, counts, (int64_t)n * p0.n_dist);
   return SF_OK;
 }
 
@@ -1075,7 +1105,7 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
   HIPCHK(hipSetDevice(ctx->device));
   const sf_problem &p0 = sl[0];
   const bool J = p0.n_val > 0, mixed = p0.precision == SF_PREC_MIXED;
-  const int ncell = p0.n_dist * (J ? p0.n_val : 1);
+  const int ncell = out_cells(p0);
   DevBuf<double> ts;
   DevBuf<unsigned long long> tc;
   HIPCHK(ts.ensure((size_t)n * ncell));
@@ -1129,6 +1159,7 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
   Pk.op = p0.op;
   Pk.nl = p0.nl;
   Pk.nt = p0.nt;
+  Pk.nmom = p0.op == SF_OP_SINGLE_PASS ? 4 : 1;
   const int G = p0.geometry == SF_GEOM_FLAT ? kFlat : kSphere;
   ctx->tuned[std::make_tuple(G, p0.D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val)] = best_c;
   if (out) std::tie(out[0], out[1], out[2], out[3]) = best_c;

@@ -98,25 +98,30 @@ struct Params {
   int32_t order;           // operator order (value-error bound exponent)
   double radius;
   float vcoef;             // binary32 value error coefficient: |dv| <= vcoef * Umax^order * amp
-  float pad1;
+  int32_t nmom;            // accumulated moments per bin: 1, or 4 for the single-pass invariants
 };
+
+// Single-pass output: (6, n_dist) per slice, rows S2 L2 T2 S3 L3 L1T2; the kernels accumulate the
+// moments S2, L2, S3, L3 (rows 0, 1, 3, 4) and the count (row 0).
+inline constexpr int kSpRows = 6;
+inline constexpr __host__ __device__ int sp_row(int k) { return k < 2 ? k : k + 1; }
 
 // LDS layout of the histogram kernels, shared by the host (sizing) and the device (carving).
 struct SmemLayout {
   int32_t hs, hc, tab, dE, vE, vE32, stage, total;  // byte offsets
 };
 
-// LDS histogram slots: joint histograms are stored value-bin fastest with an odd row stride per
+// LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each): joint histograms are stored value-bin fastest with an odd row stride per
 // distance bin (lanes of one wave usually share a distance bin but not a value bin, so their
 // atomics then fall in different banks); 1D histograms keep one copy per lane group.
 inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
 
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
-                                                  int stage_bytes, int copies) {
+                                                  int stage_bytes, int copies, int nmom) {
   SmemLayout L;
   const int nh = (n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies;
   int o = 0;
-  L.hs = o; o += 8 * nh;
+  L.hs = o; o += 8 * nh * nmom;
   L.hc = o; o += 4 * ((nh + 3) & ~3);
   L.tab = o; o += 16 * ncell_max;
   L.dE = o; o += 8 * ((n_dist + 2) & ~1);

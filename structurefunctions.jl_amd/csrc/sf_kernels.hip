@@ -189,9 +189,16 @@ DEV int cell64(double key, const Slice &s) {
   return (int)c;
 }
 
-// number of exact distance-key edges strictly below key (key > 0), 0 for key <= 0, nE for NaN.
+// number of exact distance-key edges strictly below key (nE for NaN, which sorts last).  Keys <= 0
+// (coincident points, r^2 = 0) are counted against the edges directly: a leading -Inf edge
+// (InfPaddedBinEdges) lies below them.
 DEV int dist_bin64(double key, const Cell *tab, const double *E, int nE, const Slice &s) {
-  if (!(key > 0.0)) return (key != key) ? nE : 0;
+  if (!(key > 0.0)) {
+    if (key != key) return nE;
+    int b = 0;
+    while (b < nE && E[b] < key) ++b;
+    return b;
+  }
   int b = tab[cell64(key, s)].bx;
   while (b < nE && E[b] < key) ++b;
   return b;
@@ -224,7 +231,9 @@ DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
 // Full exact evaluation of pair (i, j) of slice s from binary64 records.  Returns the output cell
 // (0-based within the slice) or -1 if the pair is not counted; writes the value.  OPK fixes the
 // operator at compile time (1: {1,0}, 2: {2,0}) with the same operation sequence as the generic
-// functor; 0 dispatches on P.op at run time.
+// functor; 0 dispatches on P.op at run time; 3 is the single pass, which writes du_L to val[0] and
+// |du|^2 to val[1] (flat: _pf_sp_simd_pairs!, serial_single_pass.jl:245-253, du_L =
+// dot(du, dx) * inv(sqrt(r2)); sphere: _sp1d_pairs!, :333-338, du_L = dot(du, e1)).
 template <int G, int D, bool J, int OPK = 0>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
                    const Cell *tab, const double *dE, const double *vE, double *val) {
@@ -241,7 +250,12 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     const double r2 = sdot<D>(dx, dx);
     db = dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
-    if constexpr (OPK == 2) {
+    if constexpr (OPK == 3) {
+      const double inv_r = 1.0 / __builtin_sqrt(r2);
+      val[0] = sdot<D>(du, dx) * inv_r;
+      val[1] = sdot<D>(du, du);
+      return db - 1;
+    } else if constexpr (OPK == 2) {
       const double p = sdot<D>(du, dx);
       v = p * p / r2;
     } else if constexpr (OPK == 1) {
@@ -260,7 +274,11 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     double du[3];
     sphere_delta64<D>(f, A, B, du);
     const double rh[3] = {1.0, 0.0, 0.0};
-    if constexpr (OPK == 1) v = 1.0 * sdot<D>(du, rh);
+    if constexpr (OPK == 3) {
+      val[0] = sdot<D>(du, rh);
+      val[1] = sdot<D>(du, du);
+      return db - 1;
+    } else if constexpr (OPK == 1) v = 1.0 * sdot<D>(du, rh);
     else if constexpr (OPK == 2) {
       const double L = sdot<D>(du, rh);
       v = 1.0 * (L * L);
@@ -376,11 +394,13 @@ struct Smem {
   int copies;
   bool joint;
   int nd, vstride, nslots;
+  int nmom;  // sums per slot, moment-major: moment k of slot h at hs[k * nslots + h]
 };
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom);
   Smem m;
+  m.nmom = P.nmom;
   m.ncell_h = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
   m.copies = P.copies;
   m.joint = P.n_val > 0;
@@ -398,10 +418,8 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
 }
 
 DEV void zero_hist(const Smem &m) {
-  for (int c = threadIdx.x; c < m.nslots; c += blockDim.x) {
-    m.hs[c] = 0.0;
-    m.hc[c] = 0u;
-  }
+  for (int c = threadIdx.x; c < m.nslots; c += blockDim.x) m.hc[c] = 0u;
+  for (int c = threadIdx.x; c < m.nslots * m.nmom; c += blockDim.x) m.hs[c] = 0.0;
 }
 
 // LDS slot of output cell c = db0 + n_dist * vb0 (0-based bins) for this lane
@@ -410,7 +428,48 @@ DEV int hslot(const Smem &m, int db0, int vb0) {
   return db0 * m.copies + (__lane_id() & (m.copies - 1));
 }
 
+// one pair into LDS slot h: the value, or (single pass) the four moments of (du_L, |du|^2) in the
+// reference's order: |du|^2, du_L^2, du_L |du|^2, du_L du_L^2 (serial_single_pass.jl:257-264)
+template <typename T>
+DEV void hist_add(const Smem &m, int h, T v) {
+  atomicAdd(&m.hs[h], (double)v);
+  atomicAdd(&m.hc[h], 1u);
+}
+template <typename T>
+DEV void hist_add_sp(const Smem &m, int h, T duL, T dn2) {
+  const T duL2 = duL * duL;
+  atomicAdd(&m.hs[h], (double)dn2);
+  atomicAdd(&m.hs[m.nslots + h], (double)duL2);
+  atomicAdd(&m.hs[2 * m.nslots + h], (double)(duL * dn2));
+  atomicAdd(&m.hs[3 * m.nslots + h], (double)(duL * duL2));
+  atomicAdd(&m.hc[h], 1u);
+}
+
 DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
+  if (m.nmom > 1) {
+    // single pass: distance bins only, per-lane copies, rows S2 L2 S3 L3 of a (6, n_dist) output
+    for (int c = threadIdx.x; c < m.nd; c += blockDim.x) {
+      unsigned long long n = 0;
+      double v[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int k = 0; k < m.copies; ++k) {
+        const int h = c * m.copies + k;
+        n += m.hc[h];
+        m.hc[h] = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          v[q] += m.hs[q * m.nslots + h];
+          m.hs[q * m.nslots + h] = 0.0;
+        }
+      }
+      if (n) {
+        const int64_t o = s.out_off + (int64_t)kSpRows * c;
+        atomicAdd(&P.counts[o], n);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) atomicAdd(&P.sums[o + sp_row(q)], v[q]);
+      }
+    }
+    return;
+  }
   for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
     unsigned long long n = 0;
     double v = 0.0;
@@ -468,12 +527,12 @@ DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const doubl
       rj[2 * k] = q.x;
       rj[2 * k + 1] = q.y;
     }
-    double v;
-    const int c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, &v);
+    double v[2];
+    const int c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
     if (c >= 0) {
       const int h = hslot(m, c % P.n_dist, c / P.n_dist);
-      atomicAdd(&m.hs[h], v);
-      atomicAdd(&m.hc[h], 1u);
+      if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
+      else hist_add(m, h, v[0]);
     }
   }
 }
@@ -637,7 +696,10 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
                      int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
   constexpr int F = Rec<G, D>::f32;
   const int nb = P.n_dist;
-  constexpr bool needT = (OPK == 0);
+  // OPK 3 (single pass) needs du_L and |du|^2; on the sphere |du|^2 = du_L^2 + |du_T|^2
+  constexpr bool SP = (OPK == 3);
+  constexpr bool needT = (OPK == 0) || (SP && G == kSphere);
+  constexpr bool needN2 = (OPK == 0) || SP;
   if (j_begin >= j_end) return;
   float rj[F];
   load_rec<F>(col, j_begin - col_j0, rj);
@@ -680,10 +742,12 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
         for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
         const float rinv = frsq(key);
         L = p * rinv;
-        if constexpr (needT) {
+        if constexpr (needN2) {
           n2 = du[0] * du[0];
 #pragma unroll
           for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
+        }
+        if constexpr (needT) {
           // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
           // formed from the cross product so that it does not cancel when du is nearly longitudinal
           const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
@@ -733,12 +797,21 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
         // and in the transport term near antipodal pairs)
         amp = __builtin_fmaf(key, inv_s, 1.0f);
       }
-      v = op32<OPK>(P, D, L, T, tn2, n2);
+      if constexpr (SP) {
+        // moments are formed at the accumulation; all four must be finite in binary32
+        const float L2 = L * L;
+        amb = amb || !(n2 < __builtin_inff()) || !(__builtin_fabsf(L * n2) < __builtin_inff()) ||
+              !(__builtin_fabsf(L * L2) < __builtin_inff());
+      } else {
+        v = op32<OPK>(P, D, L, T, tn2, n2);
+      }
       if constexpr (J) {
         const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+        // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
+        amb = amb || !(umax >= 0x1p-30f);
         vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
         live = amb || (vb >= 1 && vb <= P.n_val);
-      } else {
+      } else if constexpr (!SP) {
         amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
       }
     }
@@ -757,8 +830,8 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
     }
     if (live && !amb) {
       const int h = hslot(m, db - 1, vb - 1);
-      atomicAdd(&m.hs[h], (double)v);
-      atomicAdd(&m.hc[h], 1u);
+      if constexpr (SP) hist_add_sp(m, h, L, n2);
+      else hist_add(m, h, v);
     }
 #pragma unroll
     for (int k = 0; k < F; ++k) rj[k] = rn[k];
@@ -836,20 +909,19 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
 // ---------------------------------------------------------------------------------------------
 // recheck: binary64 binning of queued pairs (global-memory tables, global atomics)
 // ---------------------------------------------------------------------------------------------
-template <int G, int D, bool J>
+template <int G, int D, bool J, bool SP>
 __global__ void __launch_bounds__(256) recheck(Params P) {
   // Each workgroup takes a contiguous range of the queue.  Entries of the range's first slice are
   // accumulated in an LDS histogram flushed once; entries of other slices (the queue is filled in
   // roughly slice order, so these are few) go straight to the global accumulators.
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
+  constexpr int NM = SP ? 4 : 1;
   const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
-  double *hs = (double *)smem_raw;
-  unsigned *hc = (unsigned *)(hs + nh);
-  for (int c = threadIdx.x; c < nh; c += blockDim.x) {
-    hs[c] = 0.0;
-    hc[c] = 0u;
-  }
+  double *hs = (double *)smem_raw;           // NM sums per cell, moment-major
+  unsigned *hc = (unsigned *)(hs + NM * nh);
+  for (int c = threadIdx.x; c < nh; c += blockDim.x) hc[c] = 0u;
+  for (int c = threadIdx.x; c < NM * nh; c += blockDim.x) hs[c] = 0.0;
   const unsigned long long n0 = *P.pending_count;
   const unsigned long long n = n0 < (unsigned long long)P.pending_cap ? n0 : (unsigned long long)P.pending_cap;
   const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
@@ -867,16 +939,29 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
         A[t] = P.rec64[(s.pt_off + q.i) * S + t];
         Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
       }
-      double v;
-      const int c = exact_cell<G, D, J>(P, s, A, Bv, P.cells + s.tab_off, P.dedges + s.dedge_off,
-                                        P.vedges + s.vedge_off, &v);
+      double v[2];
+      const int c = exact_cell<G, D, J, SP ? 3 : 0>(P, s, A, Bv, P.cells + s.tab_off,
+                                                    P.dedges + s.dedge_off, P.vedges + s.vedge_off, v);
       if (c >= 0) {
+        double mv[NM];
+        if constexpr (SP) {
+          const double duL2 = v[0] * v[0];
+          mv[0] = v[1];
+          mv[1] = duL2;
+          mv[2] = v[0] * v[1];
+          mv[3] = v[0] * duL2;
+        } else {
+          mv[0] = v[0];
+        }
         if (q.slice == home) {
-          atomicAdd(&hs[c], v);
+#pragma unroll
+          for (int t = 0; t < NM; ++t) atomicAdd(&hs[t * nh + c], mv[t]);
           atomicAdd(&hc[c], 1u);
         } else {
-          atomicAdd(&P.counts[s.out_off + c], 1ull);
-          atomicAdd(&P.sums[s.out_off + c], v);
+          const int64_t o = s.out_off + (SP ? (int64_t)kSpRows * c : (int64_t)c);
+          atomicAdd(&P.counts[o], 1ull);
+#pragma unroll
+          for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP ? sp_row(t) : 0)], mv[t]);
         }
       }
     }
@@ -884,8 +969,10 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
     const int64_t off = P.slices[home].out_off;
     for (int c = threadIdx.x; c < nh; c += blockDim.x) {
       if (hc[c]) {
-        atomicAdd(&P.counts[off + c], (unsigned long long)hc[c]);
-        atomicAdd(&P.sums[off + c], hs[c]);
+        const int64_t o = off + (SP ? (int64_t)kSpRows * c : (int64_t)c);
+        atomicAdd(&P.counts[o], (unsigned long long)hc[c]);
+#pragma unroll
+        for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP ? sp_row(t) : 0)], hs[t * nh + c]);
       }
     }
   }
@@ -914,6 +1001,19 @@ __global__ void merge_checked(const double *src_s, const unsigned long long *src
     dst_s[k] += src_s[k];
     dst_c[k] += src_c[k];
   }
+}
+
+// _sp1d_derive_rows! (serial_single_pass.jl:208-219) over nbins (6)-row columns: T2 = S2 - L2,
+// L1T2 = S3 - L3, count rows 2..6 = row 1.  Assignments, so it is idempotent on accumulated output.
+__global__ void derive_sp(double *s, unsigned long long *c, int64_t nbins) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  double *col = s + kSpRows * b;
+  unsigned long long *cc = c + kSpRows * b;
+  col[2] = col[0] - col[1];
+  col[5] = col[3] - col[4];
+  const unsigned long long n = cc[0];
+  for (int t = 1; t < kSpRows; ++t) cc[t] = n;
 }
 
 __global__ void zero_u64(unsigned long long *p, int64_t n) {
@@ -977,6 +1077,9 @@ hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *ps, 
 
 template <int G, int D, bool J, bool SL>
 static const void *hist32_fn(int opk) {
+  if constexpr (!J) {
+    if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL>;
+  }
   if (opk == 1) return (const void *)&hist32<G, D, J, 1, SL>;
   if (opk == 2) return (const void *)&hist32<G, D, J, 2, SL>;
   return (const void *)&hist32<G, D, J, 0, SL>;
@@ -985,6 +1088,9 @@ static const void *hist32_fn(int opk) {
 template <int G, int D, bool J>
 static const void *hist_fn(bool mixed, int opk, bool sload) {
   if (mixed) return sload ? hist32_fn<G, D, J, true>(opk) : hist32_fn<G, D, J, false>(opk);
+  if constexpr (!J) {
+    if (opk == 3) return (const void *)&hist64<G, D, false, 3>;
+  }
   if (opk == 1) return (const void *)&hist64<G, D, J, 1>;
   if (opk == 2) return (const void *)&hist64<G, D, J, 2>;
   return (const void *)&hist64<G, D, J, 0>;
@@ -1000,6 +1106,7 @@ static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl)
 }
 
 static int opk_of(const Params &P) {
+  if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
   if (P.op == 0 && P.nl == 2 && P.nt == 0) return 2;
   return 0;
@@ -1025,22 +1132,21 @@ hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Param
   return hipLaunchKernel(f, dim3(grid), dim3(block), args, smem, st);
 }
 
-template <int G, int D, bool J>
-static const void *recheck_fn() { return (const void *)&recheck<G, D, J>; }
+template <int G, int D>
+static const void *recheck_fn(bool J, bool sp) {
+  if (J) return (const void *)&recheck<G, D, true, false>;
+  return sp ? (const void *)&recheck<G, D, false, true> : (const void *)&recheck<G, D, false, false>;
+}
 
 hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st) {
+  const bool sp = P.nmom > 1;
   const void *f;
-  if (G == kFlat) {
-    if (D == 2) f = J ? recheck_fn<kFlat, 2, true>() : recheck_fn<kFlat, 2, false>();
-    else f = J ? recheck_fn<kFlat, 3, true>() : recheck_fn<kFlat, 3, false>();
-  } else {
-    if (D == 2) f = J ? recheck_fn<kSphere, 2, true>() : recheck_fn<kSphere, 2, false>();
-    else f = J ? recheck_fn<kSphere, 3, true>() : recheck_fn<kSphere, 3, false>();
-  }
+  if (G == kFlat) f = D == 2 ? recheck_fn<kFlat, 2>(J, sp) : recheck_fn<kFlat, 3>(J, sp);
+  else f = D == 2 ? recheck_fn<kSphere, 2>(J, sp) : recheck_fn<kSphere, 3>(J, sp);
   Params Pc = P;
   void *args[] = {&Pc};
   const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
-  const size_t smem = (size_t)nh * 12 + 16;
+  const size_t smem = (size_t)nh * (sp ? 36 : 12) + 16;
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   return hipLaunchKernel(f, dim3(grid), dim3(256), args, smem, st);
 }
@@ -1065,6 +1171,12 @@ hipError_t launch_merge_checked(const double *src_s, const unsigned long long *s
   if (n <= 0) return hipSuccess;
   merge_checked<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src_s, src_c, dst_s, dst_c, n, pcount,
                                                              cap, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_derive_sp(double *s, unsigned long long *c, int64_t nbins, hipStream_t st) {
+  if (nbins <= 0) return hipSuccess;
+  derive_sp<<<(unsigned)((nbins + 255) / 256), 256, 0, st>>>(s, c, nbins);
   return hipGetLastError();
 }
 

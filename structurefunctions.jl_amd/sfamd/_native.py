@@ -6,7 +6,9 @@ if it cannot be loaded, every entry point raises BackendError.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+import sys
 
 import numpy as np
 
@@ -16,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
 
 SF_OK, SF_E_ARG, SF_E_DIM, SF_E_COUNT, SF_E_DEVICE, SF_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED = 0, 1
+OP_SINGLE_PASS = 6
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_check", "sf_eval_math",
@@ -38,6 +41,28 @@ class SfProblem(ctypes.Structure):
 _lib = None
 
 
+def _share_torch_hip_runtime():
+    """Make this library bind to the HIP runtime PyTorch-ROCm uses, when torch is installed.
+
+    torch ships its own libamdhip64 / libhsa-runtime64 (same sonames as /opt/rocm's) and links them
+    by file name.  If this library were loaded first, the process would hold /opt/rocm's runtime and
+    torch would then load its own copy, which cannot open the GPU a second time.  Loading torch's
+    copy first makes both resolve to one runtime whatever the import order.  Set
+    SFMI355X_OWN_HIP_RUNTIME=1 to skip this (then import torch before this library).
+    """
+    if os.environ.get("SFMI355X_OWN_HIP_RUNTIME") or "torch" in sys.modules:
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    hip = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(hip):
+        ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
 def lib():
     """Load the library once; raise BackendError if it is missing."""
     global _lib
@@ -45,6 +70,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise BackendError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C csrc")
+    _share_torch_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
     L.sf_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]

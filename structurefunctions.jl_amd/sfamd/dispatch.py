@@ -145,7 +145,7 @@ def _run(sf, slices, dedges_list, vedges_list, metric, backend):
                 for (xs, us), de, ve in zip(slices, dedges_list, vedges_list)]
     nd = len(dedges_list[0]) - 1
     nv = 0 if vedges_list[0] is None else len(vedges_list[0]) - 1
-    ncell = nd * max(nv, 1)
+    ncell = 6 * nd if sf.code[0] == _native.OP_SINGLE_PASS else nd * max(nv, 1)
     sums = np.zeros(len(slices) * ncell, dtype=np.float64)
     counts = np.zeros(len(slices) * ncell, dtype=np.uint64)
     backend.context().hist(problems, sums, counts)

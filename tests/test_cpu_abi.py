@@ -65,6 +65,17 @@ def test_check_rejects(kw, code):
     assert msg
 
 
+def test_single_pass_checks():
+    """SF_OP_SINGLE_PASS is accepted for distance bins and refused with value bins."""
+    p, keep = prob(op=_native.OP_SINGLE_PASS, nl=0)
+    assert _native.check_problems([p])[0] == 0
+    ve = np.linspace(-1, 1, 5)
+    keep.append(ve)
+    p.n_val, p.val_edges = 4, ve.ctypes.data
+    rc, msg = _native.check_problems([p])
+    assert rc == _native.SF_E_UNSUPPORTED and "single-pass" in msg
+
+
 def test_count_width_guard():
     """UInt32 counts: N = 92682 passes, 92683 is rejected (test/test_core_correctness.jl:225-249)."""
     p, keep = prob(N=92682)

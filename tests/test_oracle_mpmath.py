@@ -1,6 +1,8 @@
 """Brute force at small N in 40-digit arithmetic from independent formulas (mp_reference.py)
 pins the oracle's operators, geometry and binning: counts identical, sums within 1e-13 of
 sum |value|."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -63,3 +65,27 @@ def test_single_pass_vs_mpmath_and_single_operators(geometry, D):
         assert np.all(np.abs(s[r] - ms) <= 1e-12 * np.maximum(ma, 1e-300) + 1e-300), r
     np.testing.assert_array_equal(s[2], s[0] - s[1])
     np.testing.assert_array_equal(s[5], s[3] - s[4])
+
+
+def test_single_pass_accumulates_then_derives():
+    """calculate_structure_functions_single_pass! semantics (serial_single_pass.jl:404-421,
+    208-219): a second call adds rows S2, L2, S3, L3 and count row 1 and re-derives the rest, so
+    a buffer whose derived rows were tampered with is repaired."""
+    x, u, edges = case("flat", 2, 12, seed=77)
+    s1, c1 = _oracle.single_pass(x, u, edges)
+    p, keep = _oracle.make(x, u, edges, (1, 0, 0), "flat", 1.0, None)
+    nd = p.n_dist
+    s = np.zeros(6 * nd)
+    c = np.zeros(6 * nd, dtype=np.uint64)
+    s[2::6] = 123.0
+    c[3::6] = 7
+    for _ in range(2):
+        assert _oracle.lib().sfo_hist_single_pass(ctypes.byref(p), 1, 0, p.N, 1, s.ctypes.data, c.ctypes.data) == 0
+    s = s.reshape((6, nd), order="F")
+    c = c.reshape((6, nd), order="F")
+    for r in (0, 1, 3, 4):
+        np.testing.assert_allclose(s[r], 2 * s1[r], rtol=1e-15)
+    np.testing.assert_array_equal(s[2], s[0] - s[1])
+    np.testing.assert_array_equal(s[5], s[3] - s[4])
+    for r in range(6):
+        np.testing.assert_array_equal(c[r], 2 * c1[0])
