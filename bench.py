@@ -7,8 +7,11 @@ linear du_L bins over [-4s, 4s].  One step = one sf_hist_device call over all N(
 with the inputs already resident in HBM; the default path is the mixed-precision kernel.
 
 Prints one JSON line (metric Gpairs/s) with a live roofline entry for the dominant kernel (HIP
-events around its launch, inside the library) and a CPU baseline (the C++ restatement of the
-reference's SerialBackend/ThreadedBackend, timed on a bounded sample of the same workload).
+events around its launch, recorded by the library on the call's stream) and a CPU baseline (the
+C++ restatement of the reference's SerialBackend/ThreadedBackend, timed on a bounded sample of the
+same workload).  The HBM traffic of that kernel comes from two rocprofv3 counter runs (FETCH_SIZE,
+WRITE_SIZE) of this script with --copy-probe, scaled by the counters' reading for a plain copy of
+known size; the figures are kept in TRAFFIC_FILE and read back here.
 """
 import argparse
 import json
@@ -26,6 +29,21 @@ for p in (PKG, os.path.join(ROOT, "tests")):
 # algorithmic pair computation, per workload (derivation in DESIGN.md)
 FLOPS_PER_PAIR = {"sphere_joint_L1": 60, "flat2_L2": 28}
 PEAK_FP32_TFLOPS = 157.3
+METRIC = "Gpairs/s + % FP32 VALU roofline; mixed-prec vs FP64 GPU path at fixed error"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hist32_sphere_joint_N262144_traffic.txt")
+COPY_PROBE_BYTES = 1 << 30
+
+
+def read_traffic(path):
+    """'traffic_bytes_per_launch: <int>' from the counter summary, or None."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -38,6 +56,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64 comparison step")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one device copy of 1 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding "
@@ -97,14 +117,45 @@ def main():
 
     extra = {}
     if not args.no_fp64 and args.precision != "fp64":
+        # binary64 kernel on the same inputs: throughput, and the mixed result's error against it
+        sums.zero_()
+        counts.zero_()
+        step(args.precision)
+        torch.cuda.synchronize()
+        s32, c32 = sums.cpu().numpy().copy(), counts.cpu().numpy().copy()
+        sums.zero_()
+        counts.zero_()
         step("fp64")
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        step("fp64")
+        s64, c64 = sums.cpu().numpy().copy(), counts.cpu().numpy().copy()
+        t64s = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            step("fp64")
+            torch.cuda.synchronize()
+            t64s.append(time.perf_counter() - t1)
+        t64 = min(t64s)
+        g64 = pairs / t64 / 1e9
+        extra["fp64_gpairs_s"] = round(g64, 3)
+        extra["speedup_vs_fp64"] = round(gps / g64, 3)
+        # du_L is odd: bins of a joint (distance, du_L) histogram hold values of one sign, so the
+        # per-bin relative error |s32 - s64| / |s64| is well conditioned
+        big = c64 >= 1000
+        rel = np.abs(s32 - s64)[big] / np.maximum(np.abs(s64[big]), 1e-300)
+        extra["vs_fp64"] = {"counts_bit_exact": bool(np.array_equal(c32, c64)),
+                            "sums_max_rel_err": float(rel.max()) if rel.size else None,
+                            "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
+                            "bins_compared": int(big.sum())}
+
+    if args.copy_probe:
+        a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
         torch.cuda.synchronize()
-        t64 = time.perf_counter() - t1
-        extra["fp64_gpairs_s"] = round(pairs / t64 / 1e9, 3)
-        extra["speedup_vs_fp64"] = round((pairs / t64 / 1e9 and gps / (pairs / t64 / 1e9)), 3)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        extra["copy_probe_bytes"] = COPY_PROBE_BYTES
+        del a, b
 
     cpu = None
     if not args.no_cpu:
@@ -125,7 +176,7 @@ def main():
                          f"C++ restatement of SerialBackend/ThreadedBackend, binary64, {threads} threads"}
 
     out = {
-        "metric": "Gpairs/s (pair-histogram throughput) + % FP32 VALU roofline",
+        "metric": METRIC,
         "value": round(gps, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
@@ -133,9 +184,13 @@ def main():
         "data": "synthetic (seeded Fourier-mode field on the sphere)",
         "config": {"workload": f"sphere N={N} Haversine(1) joint du_L x distance 64x96, {args.precision}",
                    "pairs_per_step": pairs, "precision": args.precision, "rebinned_pairs": int(rebinned)},
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "kernel_ms": round(kern_s * 1e3, 3), "flops_per_pair": flops},
+        # compute-bound: binary32 vector arithmetic (no matrix instructions); 157.3 TFLOP/s is the
+        # published fp32 peak, the same figure for the vector and matrix pipes
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": read_traffic(TRAFFIC_FILE), "pipe": "VALU fp32",
+                     "kernel": "hist32<sphere,D=2,joint,du_L>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "flops_per_pair": flops},
         "cpu_baseline": cpu,
     }
     out.update(extra)

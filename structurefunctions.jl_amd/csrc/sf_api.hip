@@ -684,8 +684,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   const int block = t_block ? t_block : (J ? 1024 : 256);
   const int tj = std::min(block, 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
-  // (binary32 kernel only); 0 picks the default
-  const bool sload = mixed && (t_variant == 2 || t_variant == 0);
+  // (binary32 kernel only); 0 picks the default, measured at N = 65536: scalar loads for the
+  // spherical joint histogram (206 vs 198 Gpairs/s), LDS staging elsewhere (1D: 358 vs 288)
+  const bool sload = mixed && (t_variant == 2 || (t_variant == 0 && J && G == kSphere));
   const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
   const int stage_bytes = sload ? 0 : tj * rec_bytes;
   int copies = 1;
