@@ -65,6 +65,12 @@ extern "C" {
 #define SF_PREC_FP64 0        /* plain binary64 pair arithmetic: the drop-in (counts bit-exact)  */
 #define SF_PREC_MIXED 1       /* binary32 pair arithmetic, binary64 re-binning of pairs near any
                                  bin edge (counts bit-exact), binary64 accumulation              */
+#define SF_PREC_MIXED_FIXED 2 /* as SF_PREC_MIXED, but a joint histogram whose value edges are
+                                 uniform (to 1/64 of a bin) keeps each cell as one 64-bit
+                                 fixed-point word (count + quantized sum) in LDS: counts still
+                                 bit-exact; each pair's value enters its sum with an extra absolute
+                                 error <= 2^-23 of its value bin's width.  Other shapes run as
+                                 SF_PREC_MIXED.                                                  */
 
 typedef struct sf_ctx sf_ctx;
 

@@ -40,7 +40,8 @@ hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
 hipError_t launch_derive_sp(double *s, unsigned long long *c, int64_t nbins, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
-int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, int block, size_t smem);
+int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, bool fx, int block,
+                   size_t smem);
 int hist_opk(const Params &P);
 struct PrepSliceHost {
   const double *x, *u;
@@ -162,7 +163,7 @@ namespace {
 struct Plan {
   sf_ctx *ctx = nullptr;
   int n = 0, G = 0, D = 0, geom_code = 0, ncell = 0, S64 = 0, F32 = 0;
-  bool J = false, mixed = false, sload = false, sp = false;
+  bool J = false, mixed = false, sload = false, sp = false, fx = false;
   int block = 256, tj = 256, copies = 1, grid = 1, ncell_max = 1, n_items = 0;
   int64_t total_pts = 0, max_n = 0, total_pairs = 0, cap = 0;
   Params P{};
@@ -300,8 +301,8 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
         return ctx->fail(SF_E_ARG, "value_bins must be sorted ascending");
     }
   }
-  if (p->precision != SF_PREC_FP64 && p->precision != SF_PREC_MIXED)
-    return ctx->fail(SF_E_ARG, "precision must be SF_PREC_FP64 or SF_PREC_MIXED");
+  if (p->precision != SF_PREC_FP64 && p->precision != SF_PREC_MIXED && p->precision != SF_PREC_MIXED_FIXED)
+    return ctx->fail(SF_E_ARG, "precision must be SF_PREC_FP64, SF_PREC_MIXED or SF_PREC_MIXED_FIXED");
   if (p->count_bits != 32 && p->count_bits != 64)
     return ctx->fail(SF_E_ARG, "count_bits must be 32 or 64");
   const unsigned __int128 pairs = (unsigned __int128)p->N * (unsigned __int128)(p->N > 0 ? p->N - 1 : 0) / 2;
@@ -441,6 +442,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   s.v_uniform = 0;
   s.v_negzero = 0;
   s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0;
+  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.pad0 = 0;
   if (p.n_val > 0) {
     const int nv = p.n_val;
     const double *E = p.val_edges;
@@ -459,6 +461,13 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
         // slack in t units: edge non-uniformity + rounding of v0 and 1/w in binary32
         const long double u = ldexpl(1.0L, -24);
         s.vslack = round_up_f(dev + 4.0L * u * (1.0L + fabsl((long double)v0 / w)) + 4.0L * u * nv + 1e-6L);
+        // fixed-point joint cells need every value of bin g within [-1/64, 1 + 1/64] widths of
+        // the binary32 grid point fmaf(g, vwf, v0f)
+        if (dev < 1.0L / 64) {
+          s.v_fx = 1;
+          s.vwf = (float)w;
+          s.vsf = kFxScale / s.vwf;
+        }
       }
     }
     if (!s.v_uniform) s.vslack = (float)ldexpl(1.0L, -21);  // relative slack for binary32 edges
@@ -602,7 +611,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   pl.D = p0.D;
   pl.geom_code = p0.geometry;
   pl.J = p0.n_val > 0;
-  pl.mixed = p0.precision == SF_PREC_MIXED;
+  pl.mixed = p0.precision != SF_PREC_FP64;
   pl.ncell = out_cells(p0);
   pl.sp = p0.op == SF_OP_SINGLE_PASS;
   pl.S64 = rec_d64(pl.G, pl.D);
@@ -661,6 +670,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   }
   if (vedges.empty()) { vedges.push_back(0.0); vedges32.push_back(0.0f); }
   pl.ncell_max = ncell_max;
+  // fixed-point joint cells (binary32 kernel, one operator, value edges uniform in every slice)
+  pl.fx = p0.precision == SF_PREC_MIXED_FIXED && J && !pl.sp;
+  for (int s = 0; s < n && pl.fx; ++s) pl.fx = sdesc[s].v_uniform && sdesc[s].v_fx;
 
   // ---- launch geometry: one histogram per workgroup; large joint histograms get 1024-thread
   // workgroups (one per CU, 16 waves share the table), small 1D histograms 256-thread workgroups
@@ -681,28 +693,35 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     const auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) std::tie(t_block, t_jchunk, t_bpc, t_variant) = it->second;
   }
-  const int block = t_block ? t_block : (J ? 1024 : 256);
+  // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
+  int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : 256);
   const int tj = std::min(block, 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
-  // (binary32 kernel only); 0 picks the default, measured at N = 65536: scalar loads for the
-  // spherical joint histogram (206 vs 198 Gpairs/s), LDS staging elsewhere (1D: 358 vs 288)
-  const bool sload = mixed && (t_variant == 2 || (t_variant == 0 && J && G == kSphere));
+  // (binary32 kernel only); 0 picks the default, measured at N = 65536: scalar loads for joint
+  // histograms (sphere 212 vs 204 Gpairs/s; they also keep the record out of VGPRs, which the
+  // two-workgroup fixed-point launch needs), LDS staging for 1D (sphere 432 vs 371)
+  const bool sload = mixed && (t_variant == 2 || (t_variant == 0 && J));
   const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
   const int stage_bytes = sload ? 0 : tj * rec_bytes;
   int copies = 1;
   if (!J) {
     copies = 64;
-    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom).total * 2 > ctx->max_smem)
+    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0).total * 2 > ctx->max_smem)
       copies >>= 1;
   }
-  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom);
-  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom);
+  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx);
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom, 0);
   if ((size_t)std::max(L.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
                      ncell, std::max(L.total, L64.total), ctx->max_smem);
-  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, block, (size_t)L.total);
+  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L.total);
   if (bpc < 1) bpc = 1;
+  if (!t_block && pl.fx && bpc * block < 1024) {
+    // registers allow only one 768-thread workgroup per CU: use 1024 threads instead
+    block = 1024;
+    bpc = std::max(1, hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L.total));
+  }
   if (t_bpc) bpc = std::min(bpc, t_bpc);
   const int max_blocks = bpc * ctx->n_cu;
 
@@ -716,6 +735,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     jc = std::min(jc, 64 * block);
     jchunk = (jc + block - 1) / block * block;
   }
+  // fixed-point cells: one item must stay below the flush limit (the LDS histogram is flushed
+  // between items only)
+  if (pl.fx) jchunk = std::max(1, std::min<int>(jchunk, (int)(kFxFlushLimit / block)));
   std::vector<int> order(n);
   for (int s = 0; s < n; ++s) order[s] = s;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sl[a].N > sl[b].N; });
@@ -802,6 +824,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   // |dv| <= vcoef * (|u_i| + |u_j|)^order * amp  (flat amp = 1; sphere amp = 1 + c^2 / sin(sigma))
   P.vcoef = (float)ldexp(16.0 * std::max(1, P.order), -24);
   P.nmom = nmom;
+  P.fx = pl.fx ? 1 : 0;
+  P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
   return SF_OK;
 }
 
@@ -824,7 +848,11 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
   Params P = pl.P;
   P.stage_bytes = stage;
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom);
+  if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts
+    P.fx = 0;
+    P.flush_limit = 0x7fffffffLL;
+  }
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom, P.fx);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
@@ -959,7 +987,7 @@ extern "C" int sf_hist_batch_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
     px[s] = sl[s].x;
     pu[s] = sl[s].u;
     // coordinate magnitude for the binary32 margin: read x back (O(N)) only for the mixed flat path
-    if (sl[s].precision == SF_PREC_MIXED && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+    if (sl[s].precision != SF_PREC_FP64 && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
       std::vector<double> hx((size_t)sl[s].W * sl[s].N);
       HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
@@ -1002,7 +1030,7 @@ extern "C" int sf_plan_create(sf_ctx *ctx, int32_t n, const sf_problem *sl, sf_p
   for (int s = 0; s < n; ++s) {
     px[s] = sl[s].x;
     pu[s] = sl[s].u;
-    if (sl[s].precision == SF_PREC_MIXED && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+    if (sl[s].precision != SF_PREC_FP64 && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
       // coordinate magnitude for the binary32 margin; later contents must stay within 2x of it
       std::vector<double> hx((size_t)sl[s].W * sl[s].N);
       HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
@@ -1105,7 +1133,7 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
   if (n == 0) return ctx->fail(SF_E_ARG, "nothing to tune");
   HIPCHK(hipSetDevice(ctx->device));
   const sf_problem &p0 = sl[0];
-  const bool J = p0.n_val > 0, mixed = p0.precision == SF_PREC_MIXED;
+  const bool J = p0.n_val > 0, mixed = p0.precision != SF_PREC_FP64;
   const int ncell = out_cells(p0);
   DevBuf<double> ts;
   DevBuf<unsigned long long> tc;

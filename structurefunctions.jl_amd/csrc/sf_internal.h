@@ -57,7 +57,11 @@ struct Slice {
   double v0, vinv;    // value guess t = (v - v0) * vinv
   float v0f, vinvf;   // binary32 value guess
   float vslack;       // binary32 value-axis slack in units of t (edge non-uniformity + rounding)
-  float pad0;
+  int32_t v_fx;       // value edges uniform to 1/64 of a bin: fixed-point joint accumulation allowed
+  // fixed-point joint accumulation: a value v of value bin g (0-based) is stored as the integer
+  // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
+  float vwf, vsf;
+  int32_t pad0;
 };
 
 struct Item {
@@ -99,7 +103,18 @@ struct Params {
   double radius;
   float vcoef;             // binary32 value error coefficient: |dv| <= vcoef * Umax^order * amp
   int32_t nmom;            // accumulated moments per bin: 1, or 4 for the single-pass invariants
+  int32_t fx;              // joint histogram kept as 64-bit fixed-point words (kFx* below)
+  int32_t pad2;
+  int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
 };
+
+// Fixed-point joint cell: one 64-bit word, count in bits [44, 64), sum of q in bits [0, 44).
+// q < 2^23 (value within [-1/64, 1 + 1/64] bin widths of a_g, scaled by 2^22, biased by 2^20), so
+// fewer than 2^20 updates between flushes keep both fields exact.
+inline constexpr int kFxShift = 44;
+inline constexpr float kFxScale = 4194304.0f;  // 2^22 per value-bin width
+inline constexpr float kFxBias = 1048576.0f;   // 2^20
+inline constexpr int64_t kFxFlushLimit = (1LL << 20) - 1;
 
 // Single-pass output: (6, n_dist) per slice, rows S2 L2 T2 S3 L3 L1T2; the kernels accumulate the
 // moments S2, L2, S3, L3 (rows 0, 1, 3, 4) and the count (row 0).
@@ -115,14 +130,17 @@ struct SmemLayout {
 // distance bin (lanes of one wave usually share a distance bin but not a value bin, so their
 // atomics then fall in different banks); 1D histograms keep one copy per lane group.
 inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
+inline constexpr int kDummySlots = 64;
 
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
-                                                  int stage_bytes, int copies, int nmom) {
+                                                  int stage_bytes, int copies, int nmom, int fx) {
   SmemLayout L;
-  const int nh = (n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies;
+  // + kDummySlots: one scratch slot per lane, target of unconditional atomics of lanes that have
+  // nothing to add (never flushed)
+  const int nh = ((n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies) + kDummySlots;
   int o = 0;
   L.hs = o; o += 8 * nh * nmom;
-  L.hc = o; o += 4 * ((nh + 3) & ~3);
+  L.hc = o; o += fx ? 0 : 4 * ((nh + 3) & ~3);  // fixed-point cells carry their count
   L.tab = o; o += 16 * ncell_max;
   L.dE = o; o += 8 * ((n_dist + 2) & ~1);
   L.vE = o; o += 8 * ((n_val + 2) & ~1);

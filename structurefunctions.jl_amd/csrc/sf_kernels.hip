@@ -385,7 +385,9 @@ __global__ void prep32(const PrepSlice *ps, const double *rec64, float *rec32) {
 struct Smem {
   double *hs;
   unsigned *hc;
-  Cell *tab;
+  Cell *tab;      // binary64 kernel: cell table (bx)
+  float2 *lohi;   // binary32 kernel: ambiguity interval per cell ...
+  int *blo;       // ... and bin count below it (same LDS region as tab, structure of arrays)
   double *dE;
   double *vE;
   float *vE32;
@@ -394,22 +396,30 @@ struct Smem {
   int copies;
   bool joint;
   int nd, vstride, nslots;
-  int nmom;  // sums per slot, moment-major: moment k of slot h at hs[k * nslots + h]
+  int nmom;     // sums per slot, moment-major: moment k of slot h at hs[k * mstride + h]
+  int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
+  bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
+  unsigned long long *hq;
 };
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx);
   Smem m;
   m.nmom = P.nmom;
+  m.fx = P.fx != 0;
   m.ncell_h = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
   m.copies = P.copies;
   m.joint = P.n_val > 0;
   m.nd = P.n_dist;
   m.vstride = P.n_val > 0 ? joint_stride(P.n_val) : 0;
   m.nslots = P.n_val > 0 ? P.n_dist * m.vstride : P.n_dist * P.copies;
+  m.mstride = m.nslots + kDummySlots;
   m.hs = (double *)(base + L.hs);
+  m.hq = (unsigned long long *)(base + L.hs);
   m.hc = (unsigned *)(base + L.hc);
   m.tab = (Cell *)(base + L.tab);
+  m.lohi = (float2 *)(base + L.tab);
+  m.blo = (int *)(base + L.tab + 8 * P.ncell_max);
   m.dE = (double *)(base + L.dE);
   m.vE = (double *)(base + L.vE);
   m.vE32 = (float *)(base + L.vE32);
@@ -418,8 +428,9 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
 }
 
 DEV void zero_hist(const Smem &m) {
-  for (int c = threadIdx.x; c < m.nslots; c += blockDim.x) m.hc[c] = 0u;
-  for (int c = threadIdx.x; c < m.nslots * m.nmom; c += blockDim.x) m.hs[c] = 0.0;
+  if (!m.fx)
+    for (int c = threadIdx.x; c < m.mstride; c += blockDim.x) m.hc[c] = 0u;
+  for (int c = threadIdx.x; c < m.mstride * m.nmom; c += blockDim.x) m.hs[c] = 0.0;
 }
 
 // LDS slot of output cell c = db0 + n_dist * vb0 (0-based bins) for this lane
@@ -439,13 +450,31 @@ template <typename T>
 DEV void hist_add_sp(const Smem &m, int h, T duL, T dn2) {
   const T duL2 = duL * duL;
   atomicAdd(&m.hs[h], (double)dn2);
-  atomicAdd(&m.hs[m.nslots + h], (double)duL2);
-  atomicAdd(&m.hs[2 * m.nslots + h], (double)(duL * dn2));
-  atomicAdd(&m.hs[3 * m.nslots + h], (double)(duL * duL2));
+  atomicAdd(&m.hs[m.mstride + h], (double)duL2);
+  atomicAdd(&m.hs[2 * m.mstride + h], (double)(duL * dn2));
+  atomicAdd(&m.hs[3 * m.mstride + h], (double)(duL * duL2));
   atomicAdd(&m.hc[h], 1u);
 }
 
 DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
+  if (m.fx) {
+    // fixed-point joint cells: n = word >> 44, sum v = n a_g + (sum q - n kFxBias) / vsf
+    for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
+      const int db0 = c % m.nd, vb0 = c / m.nd;
+      const int h = vb0 + m.vstride * db0;
+      const unsigned long long wd = m.hq[h];
+      m.hq[h] = 0ull;
+      const unsigned long long n = wd >> kFxShift;
+      if (n) {
+        const unsigned long long sq = wd & ((1ull << kFxShift) - 1ull);
+        const double a = (double)__builtin_fmaf((float)vb0, s.vwf, s.v0f);
+        const double v = (double)n * a + ((double)sq - (double)n * (double)kFxBias) / (double)s.vsf;
+        atomicAdd(&P.counts[s.out_off + c], n);
+        atomicAdd(&P.sums[s.out_off + c], v);
+      }
+    }
+    return;
+  }
   if (m.nmom > 1) {
     // single pass: distance bins only, per-lane copies, rows S2 L2 S3 L3 of a (6, n_dist) output
     for (int c = threadIdx.x; c < m.nd; c += blockDim.x) {
@@ -457,8 +486,8 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
         m.hc[h] = 0u;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          v[q] += m.hs[q * m.nslots + h];
-          m.hs[q * m.nslots + h] = 0.0;
+          v[q] += m.hs[q * m.mstride + h];
+          m.hs[q * m.mstride + h] = 0.0;
         }
       }
       if (n) {
@@ -498,7 +527,15 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
 DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32) {
   const Cell *src = P.cells + (fast32 ? s.tab32_off : s.tab_off);
   const int nc = fast32 ? s.ncell32 : s.ncell64;
-  for (int c = threadIdx.x; c < nc; c += blockDim.x) m.tab[c] = src[c];
+  if (fast32) {
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const Cell e = src[c];
+      m.lohi[c] = make_float2(e.lo, e.hi);
+      m.blo[c] = e.blo;
+    }
+  } else {
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) m.tab[c] = src[c];
+  }
   for (int k = threadIdx.x; k <= P.n_dist; k += blockDim.x) m.dE[k] = P.dedges[s.dedge_off + k];
   if (P.n_val > 0) {
     for (int k = threadIdx.x; k <= P.n_val; k += blockDim.x) {
@@ -556,7 +593,7 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
     if (it >= P.n_items) break;
     const Item w = P.items[it];
     const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
-    if (w.slice != cur || since_flush + work > 0x7fffffffull) {
+    if (w.slice != cur || since_flush + work > (unsigned long long)P.flush_limit) {
       if (cur >= 0) {
         flush_hist(m, P, s);
         __syncthreads();
@@ -606,20 +643,19 @@ DEV int cell32(float key, const Slice &s) {
 
 // value-axis bin from a binary32 value with absolute error bound dv; sets amb when undecidable.
 DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, bool &amb) {
-  if (!(__builtin_fabsf(v) < __builtin_inff())) {
-    amb = true;
-    return 0;
-  }
+  // a non-finite value is undecidable here; the lookup below still runs (on a clamped guess, no
+  // branch), its bin is ignored
+  amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   if (s.v_uniform) {
     const float t = (v - s.v0f) * s.vinvf;
     const float tc = __builtin_fminf(__builtin_fmaxf(t, -2.0f), (float)nE + 1.0f);
-    const float g = __builtin_floorf(tc);
-    const float fr = tc - g;
-    // error of t: value bound, host slack (edge non-uniformity, rounding of v0 and 1/w), and
-    // the rounding of the subtraction and product (a few ulp of |t|)
-    const float eps = dv * __builtin_fabsf(s.vinvf) + s.vslack + 4.8e-7f * __builtin_fabsf(tc);
-    amb = amb || (fr <= eps) || (fr >= 1.0f - eps);
-    return (int)g + 1;
+    // error of t: value bound (1/w > 0), host slack (edge non-uniformity, rounding of v0 and
+    // 1/w), and the rounding of the subtraction, product and the two offsets below (a few ulp of
+    // |t|).  Undecidable iff an integer lies in (tc - eps, tc + eps].
+    const float eps = __builtin_fmaf(dv, s.vinvf, __builtin_fmaf(4.8e-7f, __builtin_fabsf(tc), s.vslack));
+    const float gl = __builtin_floorf(tc - eps), gh = __builtin_floorf(tc + eps);
+    amb = amb || (gl != gh);
+    return (int)gh + 1;
   }
   int lo = 0, hi = nE;
   while (lo < hi) {
@@ -687,161 +723,206 @@ DEV void load_rec(const float4 *col, int idx, float *r) {
   }
 }
 
-// Pairs (i, j) for j in [j_begin, j_end); column j's record is col[(j - col_j0) * F/4 ...],
-// either the LDS stage or (scalar-load variant) the record array itself.  The body is written
-// without early exits so that the next column's record can be requested mid-iteration, after the
-// only LDS read whose result the iteration waits for (the distance-cell table).
-template <int G, int D, bool J, int OPK, bool MASK>
-DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
-                     int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
+// Deferred histogram update of the previous pair of this lane: slot (-1: none) and value(s).
+// It is issued after the next pair's table read: LDS operations complete in order, so the wait
+// for that read does not include the atomics' latency.
+struct PendingAdd {
+  int h;
+  float v, w;
+  unsigned q;  // fixed-point joint cells: the quantized value
+};
+
+// One pair (i, j): certified distance bin from the binary32 key, operator value, certified value
+// bin, queueing of undecidable pairs, and the deferred update.  rj is column j's record; the
+// record of column j + 1 (if j + 1 < j_end) is requested into rn.  The body has no branch on the
+// pair's own state: every pair is evaluated (out-of-range pairs are simply not counted), so the
+// wave never splits.
+template <int G, int D, bool J, int OPK, bool MASK, bool FX>
+DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i, int j,
+                const float *rj, float *rn, const float4 *col, int col_j0, int j_end, int slice_id,
+                PendingAdd &pa) {
   constexpr int F = Rec<G, D>::f32;
-  const int nb = P.n_dist;
   // OPK 3 (single pass) needs du_L and |du|^2; on the sphere |du|^2 = du_L^2 + |du_T|^2
   constexpr bool SP = (OPK == 3);
   constexpr bool needT = (OPK == 0) || (SP && G == kSphere);
   constexpr bool needN2 = (OPK == 0) || SP;
+  const int nb = P.n_dist;
+  const bool upper = !MASK || j > i;
+  bool amb, amb_d;
+  float key, L = 0.f, T = 0.f, tn2 = 0.f, n2 = 0.f, amp = 1.0f, umax;
+  int db;
+  // ---- distance key and certified bin
+  float dx[3], du[3], d[3];
+  if constexpr (G == kFlat) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
+    key = dx[0] * dx[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) key = __builtin_fmaf(dx[k], dx[k], key);
+    umax = ri[4 * D] + rj[4 * D];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = (rj[k] - ri[k]) + (rj[3 + k] - ri[3 + k]);
+    key = __builtin_fmaf(d[2], d[2], __builtin_fmaf(d[1], d[1], d[0] * d[0]));
+    umax = ri[12] + rj[12];
+  }
+  {
+    const int c = cell32(key, s);
+    const float2 lh = m.lohi[c];
+    const int blo = m.blo[c];
+    // the previous pair's update, unconditional (a lane with nothing pending adds to its scratch
+    // slot) so that the wait below is a partial one on every path
+    __builtin_amdgcn_sched_barrier(0);
+    const int hh = pa.h >= 0 ? pa.h : m.nslots + (int)__lane_id();
+    if constexpr (FX) atomicAdd(&m.hq[hh], (1ull << kFxShift) | (unsigned long long)pa.q);
+    else if constexpr (SP) hist_add_sp(m, hh, pa.v, pa.w);
+    else hist_add(m, hh, pa.v);
+    // consume the table entry here (an empty asm reading it, ordered before later memory
+    // operations): the next column's record is requested only after this point, because a
+    // scalar load still in flight would make the wait for the table read wait for it as well
+    asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
+    amb_d = (key >= lh.x) & (key <= lh.y);
+    db = (key > lh.y) ? blo + 1 : blo;
+  }
+  if (j + 1 < j_end) load_rec<F>(col, j + 1 - col_j0, rn);
+  amb = amb_d;
+  const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
+  bool vb_ok = true;
+  float v = 0.f;
+  int vb = 1;
+  if constexpr (G == kFlat) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
+    float p = du[0] * dx[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
+    const float rinv = frsq(key);
+    L = p * rinv;
+    if constexpr (needN2) {
+      n2 = du[0] * du[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
+    }
+    if constexpr (needT) {
+      // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
+      // formed from the cross product so that it does not cancel when du is nearly longitudinal
+      const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
+      if constexpr (D == 2) {
+        T = cz * rinv;
+        tn2 = T * T;
+      } else {
+        T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
+        const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
+        const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
+        tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
+      }
+    }
+    amb = amb || !(key > 1e-30f);
+  } else {
+    // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
+    //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
+    //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
+    //   du_T = w.(U_B - U_A)/sin
+    const float dp = -0.5f * key;
+    float tB[3], w[3], dU[3], sp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
+      dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
+      sp[k] = ri[k] + rj[k];
+    }
+    w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
+    w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
+    w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
+    const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
+    amb = amb || !(s2 > 1e-30f);
+    const float inv_s = frsq(s2);
+    const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
+    const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
+    L = __builtin_fmaf(dp, sU, tdU) * inv_s;
+    if constexpr (needT) {
+      T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
+      tn2 = T * T;
+      if constexpr (D == 3) {
+        const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
+        tn2 = __builtin_fmaf(dw, dw, tn2);
+      }
+      n2 = __builtin_fmaf(L, L, tn2);
+    }
+    // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)) (cancellation in t_B
+    // and in the transport term near antipodal pairs)
+    amp = __builtin_fmaf(key, inv_s, 1.0f);
+  }
+  if constexpr (SP) {
+    // moments are formed at the accumulation; all four must be finite in binary32
+    const float L2 = L * L;
+    amb = amb || !(n2 < __builtin_inff()) || !(__builtin_fabsf(L * n2) < __builtin_inff()) ||
+          !(__builtin_fabsf(L * L2) < __builtin_inff());
+  } else {
+    v = op32<OPK>(P, D, L, T, tn2, n2);
+  }
+  if constexpr (J) {
+    const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+    // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
+    amb = amb || !(umax >= 0x1p-30f);
+    vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
+    vb_ok = (unsigned)(vb - 1) < (unsigned)P.n_val;
+    if constexpr (FX) {
+      // offset within the value bin in units of 2^-22 bin widths, biased, rounded to nearest
+      const float a = __builtin_fmaf((float)(vb - 1), s.vwf, s.v0f);
+      pa.q = (unsigned)__builtin_rintf(__builtin_fmaf(v - a, s.vsf, kFxBias));
+    }
+  } else if constexpr (!SP) {
+    amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
+  }
+  // binned here: certain and in range; queued: undecidable and not certainly out of range
+  const bool add = upper & !amb & db_ok & vb_ok;
+  const bool defer = upper & amb & (amb_d | db_ok);
+  const unsigned long long mask = __ballot(defer);
+  if (mask) {
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
+    base = __shfl(base, leader);
+    if (defer) {
+      const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+      if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
+    }
+  }
+  pa.h = add ? hslot(m, db - 1, vb - 1) : -1;
+  pa.v = SP ? L : v;
+  pa.w = n2;
+}
+
+// Pairs (i, j) for j in [j_begin, j_end); column j's record is col[(j - col_j0) * F/4 ...],
+// either the LDS stage or (scalar-load variant) the record array itself.  Two columns per step
+// with alternating record buffers, so that the prefetched record never has to be copied.
+template <int G, int D, bool J, int OPK, bool MASK, bool FX>
+DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
+                     int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
+  constexpr int F = Rec<G, D>::f32;
   if (j_begin >= j_end) return;
-  float rj[F];
-  load_rec<F>(col, j_begin - col_j0, rj);
-  for (int j = j_begin; j < j_end; ++j) {
-    bool live = !MASK || j > i;
-    bool amb = false;
-    float key, L = 0.f, T = 0.f, tn2 = 0.f, n2 = 0.f, amp = 1.0f, umax;
-    int db;
-    float rn[F];
-    // ---- distance key and certified bin
-    float dx[3], du[3], d[3];
-    if constexpr (G == kFlat) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
-      key = dx[0] * dx[0];
-#pragma unroll
-      for (int k = 1; k < D; ++k) key = __builtin_fmaf(dx[k], dx[k], key);
-      umax = ri[4 * D] + rj[4 * D];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) d[k] = (rj[k] - ri[k]) + (rj[3 + k] - ri[3 + k]);
-      key = __builtin_fmaf(d[2], d[2], __builtin_fmaf(d[1], d[1], d[0] * d[0]));
-      umax = ri[12] + rj[12];
-    }
-    {
-      const Cell c = m.tab[cell32(key, s)];
-      amb = (key >= c.lo) && (key <= c.hi);
-      db = (key > c.hi) ? c.blo + 1 : c.blo;
-    }
-    if (j + 1 < j_end) load_rec<F>(col, j + 1 - col_j0, rn);
-    live = live && (amb || (db >= 1 && db <= nb));
-    float v = 0.f;
-    int vb = 1;
-    if (live) {
-      if constexpr (G == kFlat) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
-        float p = du[0] * dx[0];
-#pragma unroll
-        for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
-        const float rinv = frsq(key);
-        L = p * rinv;
-        if constexpr (needN2) {
-          n2 = du[0] * du[0];
-#pragma unroll
-          for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
-        }
-        if constexpr (needT) {
-          // signed transverse component (n = zhat x rhat) and transverse energy ||du x rhat||^2,
-          // formed from the cross product so that it does not cancel when du is nearly longitudinal
-          const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
-          if constexpr (D == 2) {
-            T = cz * rinv;
-            tn2 = T * T;
-          } else {
-            T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
-            const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
-            const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
-            tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
-          }
-        }
-        amb = amb || !(key > 1e-30f);
-      } else {
-        // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
-        //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
-        //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
-        //   du_T = w.(U_B - U_A)/sin
-        const float dp = -0.5f * key;
-        float tB[3], w[3], dU[3], sp[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
-          dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
-          sp[k] = ri[k] + rj[k];
-        }
-        w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
-        w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
-        w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
-        const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
-        amb = amb || !(s2 > 1e-30f);
-        const float inv_s = frsq(s2);
-        const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
-        const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
-        L = __builtin_fmaf(dp, sU, tdU) * inv_s;
-        if constexpr (needT) {
-          T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
-          tn2 = T * T;
-          if constexpr (D == 3) {
-            const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
-            tn2 = __builtin_fmaf(dw, dw, tn2);
-          }
-          n2 = __builtin_fmaf(L, L, tn2);
-        }
-        // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)) (cancellation in t_B
-        // and in the transport term near antipodal pairs)
-        amp = __builtin_fmaf(key, inv_s, 1.0f);
-      }
-      if constexpr (SP) {
-        // moments are formed at the accumulation; all four must be finite in binary32
-        const float L2 = L * L;
-        amb = amb || !(n2 < __builtin_inff()) || !(__builtin_fabsf(L * n2) < __builtin_inff()) ||
-              !(__builtin_fabsf(L * L2) < __builtin_inff());
-      } else {
-        v = op32<OPK>(P, D, L, T, tn2, n2);
-      }
-      if constexpr (J) {
-        const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
-        // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
-        amb = amb || !(umax >= 0x1p-30f);
-        vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
-        live = amb || (vb >= 1 && vb <= P.n_val);
-      } else if constexpr (!SP) {
-        amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
-      }
-    }
-    const bool defer = live && amb;
-    const unsigned long long mask = __ballot(defer);
-    if (mask) {
-      const int lane = __lane_id();
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader);
-      if (defer) {
-        const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-        if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
-      }
-    }
-    if (live && !amb) {
-      const int h = hslot(m, db - 1, vb - 1);
-      if constexpr (SP) hist_add_sp(m, h, L, n2);
-      else hist_add(m, h, v);
-    }
-#pragma unroll
-    for (int k = 0; k < F; ++k) rj[k] = rn[k];
+  float ra[F], rb[F];
+  load_rec<F>(col, j_begin - col_j0, ra);
+  PendingAdd pa{-1, 0.f, 0.f, 0u};
+  int j = j_begin;
+  for (; j + 1 < j_end; j += 2) {
+    pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j, ra, rb, col, col_j0, j_end, slice_id, pa);
+    pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j + 1, rb, ra, col, col_j0, j_end, slice_id, pa);
+  }
+  if (j < j_end) pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j, ra, rb, col, col_j0, j_end, slice_id, pa);
+  if (pa.h >= 0) {
+    if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
+    else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
+    else hist_add(m, pa.h, pa.v);
   }
 }
 
 // SLOAD: column records are read straight from global memory with wave-uniform addresses, which
 // the compiler turns into scalar loads (SGPR operands, scalar cache) -- no LDS staging, no
 // barriers inside an item.  Otherwise columns are staged through LDS in chunks of P.tj.
-template <int G, int D, bool J, int OPK, bool SLOAD>
+template <int G, int D, bool J, int OPK, bool SLOAD, bool FX>
 __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
@@ -860,7 +941,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
     if (it >= P.n_items) break;
     const Item w = P.items[it];
     const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
-    if (w.slice != cur || since_flush + work > 0x7fffffffull) {
+    if (w.slice != cur || since_flush + work > (unsigned long long)P.flush_limit) {
       if (cur >= 0) {
         flush_hist(m, P, s);
         __syncthreads();
@@ -881,10 +962,15 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
     for (int k = 0; k < F; ++k) ri[k] = iv ? rec[(s.pt_off + i) * F + k] : 0.0f;
     if constexpr (SLOAD) {
       if (iv) {
-        const float4 *col = (const float4 *)(rec + s.pt_off * F);
+        // the slice offset is wave-uniform by construction; say so, so that column addresses
+        // stay in SGPRs (the pointer itself keeps its global address space: scalar loads)
+        const uint64_t po = (uint64_t)s.pt_off;
+        const int64_t off = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(po >> 32)) << 32) |
+                                      (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)po));
+        const float4 *col = (const float4 *)(rec + off * F);
         const int jm = min(w.j1, w.i0 + B);  // columns that can still be <= some row of the tile
-        if (w.j0 < jm) run_chunk32<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, jm, col, 0, cur);
-        if (jm < w.j1) run_chunk32<G, D, J, OPK, false>(P, s, m, ri, i, max(jm, w.j0), w.j1, col, 0, cur);
+        if (w.j0 < jm) run_chunk32<G, D, J, OPK, true, FX>(P, s, m, ri, i, w.j0, jm, col, 0, cur);
+        if (jm < w.j1) run_chunk32<G, D, J, OPK, false, FX>(P, s, m, ri, i, max(jm, w.j0), w.j1, col, 0, cur);
       }
     } else {
       for (int jb = w.j0; jb < w.j1; jb += P.tj) {
@@ -896,8 +982,8 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
         __syncthreads();
         if (iv) {
           const float4 *col = (const float4 *)m.stage;
-          if (jb < w.i0 + B) run_chunk32<G, D, J, OPK, true>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
-          else run_chunk32<G, D, J, OPK, false>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+          if (jb < w.i0 + B) run_chunk32<G, D, J, OPK, true, FX>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+          else run_chunk32<G, D, J, OPK, false, FX>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
         }
       }
     }
@@ -1075,19 +1161,28 @@ hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *ps, 
                 : prep_impl<kSphere, 3>(mixed, geom_code, ps, n_slices, max_n, rec64, rec32, st);
 }
 
-template <int G, int D, bool J, bool SL>
+template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
   if constexpr (!J) {
-    if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL>;
+    if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
   }
-  if (opk == 1) return (const void *)&hist32<G, D, J, 1, SL>;
-  if (opk == 2) return (const void *)&hist32<G, D, J, 2, SL>;
-  return (const void *)&hist32<G, D, J, 0, SL>;
+  if (opk == 1) return (const void *)&hist32<G, D, J, 1, SL, FX>;
+  if (opk == 2) return (const void *)&hist32<G, D, J, 2, SL, FX>;
+  return (const void *)&hist32<G, D, J, 0, SL, FX>;
+}
+
+template <int G, int D, bool J, bool SL>
+static const void *hist32_fx(bool fx, int opk) {
+  // fixed-point cells exist for joint histograms only
+  if constexpr (J) {
+    if (fx) return hist32_fn<G, D, true, SL, true>(opk);
+  }
+  return hist32_fn<G, D, J, SL, false>(opk);
 }
 
 template <int G, int D, bool J>
-static const void *hist_fn(bool mixed, int opk, bool sload) {
-  if (mixed) return sload ? hist32_fn<G, D, J, true>(opk) : hist32_fn<G, D, J, false>(opk);
+static const void *hist_fn(bool mixed, int opk, bool sload, bool fx) {
+  if (mixed) return sload ? hist32_fx<G, D, J, true>(fx, opk) : hist32_fx<G, D, J, false>(fx, opk);
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist64<G, D, false, 3>;
   }
@@ -1096,13 +1191,13 @@ static const void *hist_fn(bool mixed, int opk, bool sload) {
   return (const void *)&hist64<G, D, J, 0>;
 }
 
-static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl) {
+static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl, bool fx) {
   if (G == kFlat) {
-    if (D == 2) return J ? hist_fn<kFlat, 2, true>(mixed, opk, sl) : hist_fn<kFlat, 2, false>(mixed, opk, sl);
-    return J ? hist_fn<kFlat, 3, true>(mixed, opk, sl) : hist_fn<kFlat, 3, false>(mixed, opk, sl);
+    if (D == 2) return J ? hist_fn<kFlat, 2, true>(mixed, opk, sl, fx) : hist_fn<kFlat, 2, false>(mixed, opk, sl, fx);
+    return J ? hist_fn<kFlat, 3, true>(mixed, opk, sl, fx) : hist_fn<kFlat, 3, false>(mixed, opk, sl, fx);
   }
-  if (D == 2) return J ? hist_fn<kSphere, 2, true>(mixed, opk, sl) : hist_fn<kSphere, 2, false>(mixed, opk, sl);
-  return J ? hist_fn<kSphere, 3, true>(mixed, opk, sl) : hist_fn<kSphere, 3, false>(mixed, opk, sl);
+  if (D == 2) return J ? hist_fn<kSphere, 2, true>(mixed, opk, sl, fx) : hist_fn<kSphere, 2, false>(mixed, opk, sl, fx);
+  return J ? hist_fn<kSphere, 3, true>(mixed, opk, sl, fx) : hist_fn<kSphere, 3, false>(mixed, opk, sl, fx);
 }
 
 static int opk_of(const Params &P) {
@@ -1112,8 +1207,9 @@ static int opk_of(const Params &P) {
   return 0;
 }
 
-int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, int block, size_t smem) {
-  const void *f = pick_hist(G, D, J, mixed, opk, sload);
+int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, bool fx, int block,
+                   size_t smem) {
+  const void *f = pick_hist(G, D, J, mixed, opk, sload, fx);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, smem) != hipSuccess) return 1;
@@ -1124,7 +1220,7 @@ int hist_opk(const Params &P) { return opk_of(P); }
 
 hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Params &P, int grid,
                        int block, size_t smem, hipStream_t st) {
-  const void *f = pick_hist(G, D, J, mixed, opk_of(P), sload);
+  const void *f = pick_hist(G, D, J, mixed, opk_of(P), sload, mixed && P.fx);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   Params Pc = P;
   const void *rec = mixed ? (const void *)P.rec32 : (const void *)P.rec64;

@@ -17,7 +17,7 @@ from .errors import ArgumentError, BackendError, DimensionMismatch
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "libsfmi355x.so")
 
 SF_OK, SF_E_ARG, SF_E_DIM, SF_E_COUNT, SF_E_DEVICE, SF_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_FIXED = 0, 1, 2
 OP_SINGLE_PASS = 6
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",

@@ -33,18 +33,24 @@ class MI355XBackend:
     ``precision="fp64"`` is the drop-in binary64 kernel (counts bit-identical to SerialBackend);
     ``precision="mixed"`` runs the pair arithmetic in binary32, re-bins every pair whose distance
     or value lies within the binary32 error bound of a bin edge in binary64 (counts still
-    bit-identical), and accumulates in binary64.
+    bit-identical), and accumulates in binary64.  ``precision="mixed_fixed"`` is "mixed" with
+    joint cells of uniform value bins kept as 64-bit fixed-point words on the device: counts
+    bit-identical, each pair's value entering its sum with an extra absolute error of at most
+    2^-23 of its value bin's width.
     """
 
+    _CODES = {"fp64": _native.PREC_FP64, "mixed": _native.PREC_MIXED,
+              "mixed_fixed": _native.PREC_MIXED_FIXED}
+
     def __init__(self, device: int = 0, precision: str = "fp64"):
-        if precision not in ("fp64", "mixed"):
-            raise ArgumentError("precision must be 'fp64' or 'mixed'")
+        if precision not in self._CODES:
+            raise ArgumentError("precision must be 'fp64', 'mixed' or 'mixed_fixed'")
         self.device = int(device)
         self.precision = precision
 
     @property
     def precision_code(self):
-        return _native.PREC_FP64 if self.precision == "fp64" else _native.PREC_MIXED
+        return self._CODES[self.precision]
 
     def context(self) -> _native.Context:
         ctx = _CONTEXTS.get(self.device)

@@ -42,7 +42,7 @@ def gpu_hist(ctx, x, u, dedges, op, geometry="flat", radius=1.0, vedges=None, pr
         keep.append(ve)
         p.n_val, p.val_edges = len(ve) - 1, ve.ctypes.data
         ncell = (len(de) - 1) * (len(ve) - 1)
-    p.precision = 0 if prec == "fp64" else 1
+    p.precision = {"fp64": 0, "mixed": 1, "fixed": 2}[prec]
     p.count_bits = 64
     s = np.zeros(ncell)
     c = np.zeros(ncell, dtype=np.uint64)
@@ -122,3 +122,23 @@ def test_parity_all_operators(ctx, geometry, D, joint, prec):
         gs, gc = gpu_hist(ctx, x, u, dedges, op, geometry=geometry, vedges=ve, prec=prec)
         check(gs, gc, os_, oc, oa, TOL[prec], f"{geometry} D={D} op={op} joint={joint} {prec}",
               og=og, mag=MAG[prec])
+
+
+@pytest.mark.parametrize("geometry,D", CASES)
+def test_parity_fixed_point_joint(ctx, geometry, D):
+    """SF_PREC_MIXED_FIXED: joint cells of uniform value bins as 64-bit fixed-point words.
+    Counts bit-identical; each pair adds at most 2^-23 value-bin widths of absolute error on top
+    of the mixed kernel's bound (checked with a factor 1.5 to spare)."""
+    x, u, dedges = field(geometry, D, 700, seed=31 + D)
+    for op in [(0, 1, 0), (0, 2, 0), (0, 0, 3), (1, 0, 0), (5, 1, 0)]:
+        ve = np.linspace(-3, 3, 31) if op in ((0, 1, 0), (0, 0, 3)) else np.linspace(0, 12, 37)
+        w = ve[1] - ve[0]
+        os_, oc, oa, og = _oracle.hist(x, u, dedges, op=op, geometry=geometry, vedges=ve, nthreads=8,
+                                       with_mag=True)
+        gs, gc = gpu_hist(ctx, x, u, dedges, op, geometry=geometry, vedges=ve, prec="fixed")
+        assert np.array_equal(gc, oc), f"{geometry} D={D} op={op}: counts differ"
+        ok = ~np.isnan(os_)
+        err = np.abs(gs - os_)[ok]
+        bound = (TOL["mixed"] * np.maximum(oa, 1e-300) + MAG["mixed"] * og
+                 + 1.5 * 2.0 ** -23 * w * oc.astype(float))[ok]
+        assert np.all(err <= bound), f"{geometry} D={D} op={op}: max excess {np.max(err - bound):.3g}"
