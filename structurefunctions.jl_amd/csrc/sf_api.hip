@@ -695,7 +695,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   }
   // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
   int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : 256);
-  const int tj = std::min(block, 256);
+  // column records staged per LDS chunk; the fixed-point joint launch keeps two workgroups per CU
+  const int tj = std::min(block, pl.fx ? 128 : 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
   // (binary32 kernel only); 0 picks the default, measured at N = 65536: scalar loads for joint
   // histograms (sphere 212 vs 204 Gpairs/s; they also keep the record out of VGPRs, which the

@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
     __syncthreads();
-    const int it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item and slice in SGPRs
     if (it >= P.n_items) break;
     const Item w = P.items[it];
     const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
@@ -642,20 +642,24 @@ DEV int cell32(float key, const Slice &s) {
 }
 
 // value-axis bin from a binary32 value with absolute error bound dv; sets amb when undecidable.
-DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, bool &amb) {
-  // a non-finite value is undecidable here; the lookup below still runs (on a clamped guess, no
-  // branch), its bin is ignored
+// Returns the 0-based value bin g (valid bins 0 .. nE-2; anything else is out of range) and g as
+// a float in gf.
+template <bool VU>
+DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, bool &amb, float &gf) {
+  // a non-finite value is undecidable here; the lookup below still runs, its bin is ignored
   amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
-  if (s.v_uniform) {
+  if constexpr (VU) {  // uniform value edges (s.v_uniform; chosen once per column range)
     const float t = (v - s.v0f) * s.vinvf;
-    const float tc = __builtin_fminf(__builtin_fmaxf(t, -2.0f), (float)nE + 1.0f);
     // error of t: value bound (1/w > 0), host slack (edge non-uniformity, rounding of v0 and
     // 1/w), and the rounding of the subtraction, product and the two offsets below (a few ulp of
-    // |t|).  Undecidable iff an integer lies in (tc - eps, tc + eps].
-    const float eps = __builtin_fmaf(dv, s.vinvf, __builtin_fmaf(4.8e-7f, __builtin_fabsf(tc), s.vslack));
-    const float gl = __builtin_floorf(tc - eps), gh = __builtin_floorf(tc + eps);
+    // |t|).  Undecidable iff an integer lies in (t - eps, t + eps].  Far outside the edges
+    // (|t| >= 2^21) eps exceeds 1, so such values are undecidable too (and re-binned); the
+    // float-to-int conversion saturates, so no clamp is needed.
+    const float eps = __builtin_fmaf(dv, s.vinvf, __builtin_fmaf(4.8e-7f, __builtin_fabsf(t), s.vslack));
+    const float gl = __builtin_floorf(t - eps), gh = __builtin_floorf(t + eps);
     amb = amb || (gl != gh);
-    return (int)gh + 1;
+    gf = gh;
+    return (int)gh;
   }
   int lo = 0, hi = nE;
   while (lo < hi) {
@@ -666,7 +670,8 @@ DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, b
   const float gap = dv + s.vslack * __builtin_fabsf(v) + 1e-37f;
   if (lo > 0 && v - E32[lo - 1] <= gap) amb = true;
   if (lo < nE && E32[lo] - v <= gap) amb = true;
-  return lo;
+  gf = (float)(lo - 1);
+  return lo - 1;
 }
 
 // binary32 value of the operator from du_L, du_T (signed transverse), ||du||^2 - du_L^2.
@@ -737,9 +742,9 @@ struct PendingAdd {
 // record of column j + 1 (if j + 1 < j_end) is requested into rn.  The body has no branch on the
 // pair's own state: every pair is evaluated (out-of-range pairs are simply not counted), so the
 // wave never splits.
-template <int G, int D, bool J, int OPK, bool MASK, bool FX>
+template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
 DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i, int j,
-                const float *rj, float *rn, const float4 *col, int col_j0, int j_end, int slice_id,
+                const float *rj, float *rn, const float4 *next, bool has_next, int slice_id,
                 PendingAdd &pa) {
   constexpr int F = Rec<G, D>::f32;
   // OPK 3 (single pass) needs du_L and |du|^2; on the sphere |du|^2 = du_L^2 + |du_T|^2
@@ -784,12 +789,17 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     amb_d = (key >= lh.x) & (key <= lh.y);
     db = (key > lh.y) ? blo + 1 : blo;
   }
-  if (j + 1 < j_end) load_rec<F>(col, j + 1 - col_j0, rn);
+  if (has_next) {
+    load_rec<F>(next, 0, rn);
+    // scalar-load variant: one 64-byte request per 16-float record rather than a split 8+4+1
+    // (the scalar data cache is request-bound here)
+    if constexpr (SL && F == 16) asm volatile("" ::"s"(rn[13]), "s"(rn[14]), "s"(rn[15]));
+  }
   amb = amb_d;
   const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
   bool vb_ok = true;
   float v = 0.f;
-  int vb = 1;
+  int vb0 = 0;
   if constexpr (G == kFlat) {
 #pragma unroll
     for (int k = 0; k < D; ++k) du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
@@ -819,27 +829,21 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     }
     amb = amb || !(key > 1e-30f);
   } else {
-    // geodesic frame (HelperFunctions.jl:186-200) with p.d = -c^2/2 for unit vectors:
-    //   t_B = d + dp q,  t_B - t_A = dp (p + q),  w = p x d,  sin^2(sigma) = |w|^2
-    //   du_L = (t_B.U_B - t_A.U_A)/sin = (t_B.(U_B - U_A) + dp (p + q).U_A)/sin
-    //   du_T = w.(U_B - U_A)/sin
-    const float dp = -0.5f * key;
-    float tB[3], w[3], dU[3], sp[3];
+    // geodesic frame (HelperFunctions.jl:186-200): t_A = (d - (p.d) p)/sin, t_B = (d + (p.d) q)/sin
+    // with d = q - p, sin^2(sigma) = |p x d|^2.  U_A is tangent at p and U_B at q, so
+    //   du_L = t_B.U_B - t_A.U_A = (d.(U_B - U_A) + (p.d)(q.U_B + p.U_A))/sin = d.dU / sin
+    //   du_T = (p x d).dU / sin
+    // (the binary32 error of this form stays within the bound used below, like the long form)
+    float w[3], dU[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      tB[k] = __builtin_fmaf(dp, rj[k], d[k]);
-      dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
-      sp[k] = ri[k] + rj[k];
-    }
+    for (int k = 0; k < 3; ++k) dU[k] = (rj[6 + k] - ri[6 + k]) + (rj[9 + k] - ri[9 + k]);
     w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
     w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
     w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
     const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
     amb = amb || !(s2 > 1e-30f);
     const float inv_s = frsq(s2);
-    const float sU = __builtin_fmaf(sp[2], ri[8], __builtin_fmaf(sp[1], ri[7], sp[0] * ri[6]));
-    const float tdU = __builtin_fmaf(tB[2], dU[2], __builtin_fmaf(tB[1], dU[1], tB[0] * dU[0]));
-    L = __builtin_fmaf(dp, sU, tdU) * inv_s;
+    L = __builtin_fmaf(d[2], dU[2], __builtin_fmaf(d[1], dU[1], d[0] * dU[0])) * inv_s;
     if constexpr (needT) {
       T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
       tn2 = T * T;
@@ -849,8 +853,8 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
       }
       n2 = __builtin_fmaf(L, L, tn2);
     }
-    // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)) (cancellation in t_B
-    // and in the transport term near antipodal pairs)
+    // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)): near antipodal pairs the
+    // numerators cancel and sin(sigma) is small
     amp = __builtin_fmaf(key, inv_s, 1.0f);
   }
   if constexpr (SP) {
@@ -865,11 +869,12 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
     // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
     amb = amb || !(umax >= 0x1p-30f);
-    vb = val_bin32(v, dv, s, m.vE32, P.n_val + 1, amb);
-    vb_ok = (unsigned)(vb - 1) < (unsigned)P.n_val;
+    float gf;
+    vb0 = val_bin32<VU>(v, dv, s, m.vE32, P.n_val + 1, amb, gf);
+    vb_ok = (unsigned)vb0 < (unsigned)P.n_val;
     if constexpr (FX) {
       // offset within the value bin in units of 2^-22 bin widths, biased, rounded to nearest
-      const float a = __builtin_fmaf((float)(vb - 1), s.vwf, s.v0f);
+      const float a = __builtin_fmaf(gf, s.vwf, s.v0f);
       pa.q = (unsigned)__builtin_rintf(__builtin_fmaf(v - a, s.vsf, kFxBias));
     }
   } else if constexpr (!SP) {
@@ -890,7 +895,10 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
       if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
     }
   }
-  pa.h = add ? hslot(m, db - 1, vb - 1) : -1;
+  int h;
+  if constexpr (J) h = vb0 + m.vstride * (db - 1);
+  else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
+  pa.h = add ? h : -1;
   pa.v = SP ? L : v;
   pa.w = n2;
 }
@@ -898,25 +906,38 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
 // Pairs (i, j) for j in [j_begin, j_end); column j's record is col[(j - col_j0) * F/4 ...],
 // either the LDS stage or (scalar-load variant) the record array itself.  Two columns per step
 // with alternating record buffers, so that the prefetched record never has to be copied.
-template <int G, int D, bool J, int OPK, bool MASK, bool FX>
-DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
-                     int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
+template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
+DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
+                        int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
   constexpr int F = Rec<G, D>::f32;
+  constexpr int R = F / 4;  // float4 per record
   if (j_begin >= j_end) return;
   float ra[F], rb[F];
-  load_rec<F>(col, j_begin - col_j0, ra);
+  // cp walks the column records two at a time; the loads use constant offsets from it
+  const float4 *cp = col + (j_begin - col_j0) * R;
+  load_rec<F>(cp, 0, ra);
   PendingAdd pa{-1, 0.f, 0.f, 0u};
   int j = j_begin;
-  for (; j + 1 < j_end; j += 2) {
-    pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j, ra, rb, col, col_j0, j_end, slice_id, pa);
-    pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j + 1, rb, ra, col, col_j0, j_end, slice_id, pa);
+  for (; j + 1 < j_end; j += 2, cp += 2 * R) {
+    pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, true, slice_id, pa);
+    pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j + 1, rb, ra, cp + 2 * R, j + 2 < j_end, slice_id, pa);
   }
-  if (j < j_end) pair32<G, D, J, OPK, MASK, FX>(P, s, m, ri, i, j, ra, rb, col, col_j0, j_end, slice_id, pa);
+  if (j < j_end) pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, false, slice_id, pa);
   if (pa.h >= 0) {
     if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
     else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
     else hist_add(m, pa.h, pa.v);
   }
+}
+
+// the value-edge lookup (uniform or searched) is chosen once per column range, not per pair
+template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool SL>
+DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
+                     int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
+  if (FX || (J && s.v_uniform))
+    run_chunk32_vu<G, D, J, OPK, MASK, FX, true, SL>(P, s, m, ri, i, j_begin, j_end, col, col_j0, slice_id);
+  else
+    run_chunk32_vu<G, D, J, OPK, MASK, FX, false, SL>(P, s, m, ri, i, j_begin, j_end, col, col_j0, slice_id);
 }
 
 // SLOAD: column records are read straight from global memory with wave-uniform addresses, which
@@ -937,7 +958,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
     __syncthreads();
-    const int it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: item and slice in SGPRs
     if (it >= P.n_items) break;
     const Item w = P.items[it];
     const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
@@ -969,8 +990,8 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
                                       (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)po));
         const float4 *col = (const float4 *)(rec + off * F);
         const int jm = min(w.j1, w.i0 + B);  // columns that can still be <= some row of the tile
-        if (w.j0 < jm) run_chunk32<G, D, J, OPK, true, FX>(P, s, m, ri, i, w.j0, jm, col, 0, cur);
-        if (jm < w.j1) run_chunk32<G, D, J, OPK, false, FX>(P, s, m, ri, i, max(jm, w.j0), w.j1, col, 0, cur);
+        if (w.j0 < jm) run_chunk32<G, D, J, OPK, true, FX, true>(P, s, m, ri, i, w.j0, jm, col, 0, cur);
+        if (jm < w.j1) run_chunk32<G, D, J, OPK, false, FX, true>(P, s, m, ri, i, max(jm, w.j0), w.j1, col, 0, cur);
       }
     } else {
       for (int jb = w.j0; jb < w.j1; jb += P.tj) {
@@ -982,8 +1003,8 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
         __syncthreads();
         if (iv) {
           const float4 *col = (const float4 *)m.stage;
-          if (jb < w.i0 + B) run_chunk32<G, D, J, OPK, true, FX>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
-          else run_chunk32<G, D, J, OPK, false, FX>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+          if (jb < w.i0 + B) run_chunk32<G, D, J, OPK, true, FX, false>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
+          else run_chunk32<G, D, J, OPK, false, FX, false>(P, s, m, ri, i, jb, jb + cnt, col, jb, cur);
         }
       }
     }
