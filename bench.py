@@ -4,7 +4,10 @@ Workload (default): the spherical joint-2D configuration -- N = 262144 points on
 (half uniform, half on 512 great-circle tracks), Haversine(R=1), operator du_L
 (ProjectedStructureFunctionType{1,0}), joint histogram of 64 log distance bins [1e-5, pi] x 96
 linear du_L bins over [-4s, 4s].  One step = one sf_hist_device call over all N(N-1)/2 pairs
-with the inputs already resident in HBM; the default path is the mixed-precision kernel.
+with the inputs already resident in HBM.  The default precision is "mixed_fixed": binary32 pair
+arithmetic, binary64 re-binning of pairs near any bin edge (counts bit-exact), joint cells kept
+in LDS as 64-bit fixed-point words and flushed to binary64.  The strict "mixed" and the binary64
+kernels are timed on the same inputs, and the per-bin error against binary64 is reported.
 
 Prints one JSON line (metric Gpairs/s) with a live roofline entry for the dominant kernel (HIP
 events around its launch, recorded by the library on the call's stream) and a CPU baseline (the
@@ -29,8 +32,13 @@ for p in (PKG, os.path.join(ROOT, "tests")):
 # algorithmic pair computation, per workload (derivation in DESIGN.md)
 FLOPS_PER_PAIR = {"sphere_joint_L1": 60, "flat2_L2": 28}
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 METRIC = "Gpairs/s + % FP32 VALU roofline; mixed-prec vs FP64 GPU path at fixed error"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hist32_sphere_joint_N262144_traffic.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hist32_sphere_joint_fixed_N262144_traffic.txt")
+PREC_CODE = {"fp64": 0, "mixed": 1, "mixed_fixed": 2}
+DTYPE = {"fp64": "f64",
+         "mixed": "f32 pair arithmetic, f64 re-binning and accumulation",
+         "mixed_fixed": "f32 pair arithmetic, f64 re-binning, 64-bit fixed-point LDS cells, f64 accumulation"}
 COPY_PROBE_BYTES = 1 << 30
 
 
@@ -51,7 +59,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--precision", choices=["mixed_fixed", "mixed", "fp64"], default="mixed_fixed")
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64 comparison step")
@@ -90,7 +98,7 @@ def main():
         p.op, p.nl, p.nt = 0, 1, 0
         p.n_dist, p.dist_edges = nd, dedges.ctypes.data
         p.n_val, p.val_edges = nv, vedges.ctypes.data
-        p.precision = 0 if prec == "fp64" else 1
+        p.precision = PREC_CODE[prec]
         p.count_bits = 64
         return p
 
@@ -113,40 +121,48 @@ def main():
     kern_s = float(np.mean(kern_ns)) * 1e-9
     flops = FLOPS_PER_PAIR["sphere_joint_L1"]
     achieved = pairs * flops / kern_s / 1e12
+    peak = PEAK_FP64_TFLOPS if args.precision == "fp64" else PEAK_FP32_TFLOPS
     rebinned = st["rebinned"]
+
+    def rate(prec, reps=2):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            step(prec)
+            torch.cuda.synchronize()
+            dt1 = time.perf_counter() - t1
+            best = dt1 if best is None else min(best, dt1)
+        return pairs / best / 1e9
+
+    def result(prec):
+        sums.zero_()
+        counts.zero_()
+        step(prec)
+        torch.cuda.synchronize()
+        return sums.cpu().numpy().copy(), counts.cpu().numpy().copy()
 
     extra = {}
     if not args.no_fp64 and args.precision != "fp64":
-        # binary64 kernel on the same inputs: throughput, and the mixed result's error against it
-        sums.zero_()
-        counts.zero_()
-        step(args.precision)
-        torch.cuda.synchronize()
-        s32, c32 = sums.cpu().numpy().copy(), counts.cpu().numpy().copy()
-        sums.zero_()
-        counts.zero_()
-        step("fp64")
-        torch.cuda.synchronize()
-        s64, c64 = sums.cpu().numpy().copy(), counts.cpu().numpy().copy()
-        t64s = []
-        for _ in range(2):
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            step("fp64")
-            torch.cuda.synchronize()
-            t64s.append(time.perf_counter() - t1)
-        t64 = min(t64s)
-        g64 = pairs / t64 / 1e9
+        # binary64 kernel on the same inputs: throughput, and each binary32 result's error against it
+        s64, c64 = result("fp64")
+        g64 = rate("fp64")
         extra["fp64_gpairs_s"] = round(g64, 3)
         extra["speedup_vs_fp64"] = round(gps / g64, 3)
         # du_L is odd: bins of a joint (distance, du_L) histogram hold values of one sign, so the
         # per-bin relative error |s32 - s64| / |s64| is well conditioned
         big = c64 >= 1000
-        rel = np.abs(s32 - s64)[big] / np.maximum(np.abs(s64[big]), 1e-300)
-        extra["vs_fp64"] = {"counts_bit_exact": bool(np.array_equal(c32, c64)),
-                            "sums_max_rel_err": float(rel.max()) if rel.size else None,
-                            "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
-                            "bins_compared": int(big.sum())}
+        precs = [args.precision] + (["mixed"] if args.precision == "mixed_fixed" else [])
+        for prec in precs:
+            s32, c32 = result(prec)
+            rel = np.abs(s32 - s64)[big] / np.maximum(np.abs(s64[big]), 1e-300)
+            extra[f"{prec}_vs_fp64"] = {
+                "counts_bit_exact": bool(np.array_equal(c32, c64)),
+                "sums_max_rel_err": float(rel.max()) if rel.size else None,
+                "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
+                "bins_compared": int(big.sum())}
+        if args.precision == "mixed_fixed":
+            extra["mixed_gpairs_s"] = round(rate("mixed"), 3)
 
     if args.copy_probe:
         a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
@@ -180,16 +196,18 @@ def main():
         "value": round(gps, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 pair arithmetic, f64 re-binning and accumulation" if args.precision == "mixed" else "f64",
+        "dtype": DTYPE[args.precision],
         "data": "synthetic (seeded Fourier-mode field on the sphere)",
         "config": {"workload": f"sphere N={N} Haversine(1) joint du_L x distance 64x96, {args.precision}",
                    "pairs_per_step": pairs, "precision": args.precision, "rebinned_pairs": int(rebinned)},
         # compute-bound: binary32 vector arithmetic (no matrix instructions); 157.3 TFLOP/s is the
         # published fp32 peak, the same figure for the vector and matrix pipes
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": read_traffic(TRAFFIC_FILE), "pipe": "VALU fp32",
-                     "kernel": "hist32<sphere,D=2,joint,du_L>", "kernel_ms": round(kern_s * 1e3, 3),
+                     "kernel": "hist32<sphere,D=2,joint,du_L" + (",fixed-point cells>" if args.precision == "mixed_fixed" else ">")
+                               if args.precision != "fp64" else "hist64<sphere,D=2,joint,du_L>",
+                     "kernel_ms": round(kern_s * 1e3, 3),
                      "flops_per_pair": flops},
         "cpu_baseline": cpu,
     }
