@@ -5,7 +5,10 @@
 #   _dispatch_execution_backend(backend, shape::PointField{D}, sf, x, u, distance_bins, value_bins; ...)
 #   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins; ...)
 #   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins, value_bins; ...)
-# (src/Calculations/dispatch.jl:377-477, 510-594).  The public entry points validate shapes and
+# (src/Calculations/dispatch.jl:377-477, 510-594), and the two single-pass hooks
+#   _dispatch_single_pass(backend, shape::PointField, x, u, distance_bins; ...)
+#   _dispatch_single_pass!(backend, sums, counts, x, u, distance_bins; ...)
+# (src/Calculations/serial_single_pass.jl:400-535).  The public entry points validate shapes and
 # count width and apply `_finalize` around these calls (dispatch.jl:64-136, 346-373), exactly as
 # for the Serial/Threaded/GPU backends.
 #
@@ -33,7 +36,9 @@ const libsfmi355x = get(ENV, "SFMI355X_LIB", joinpath(@__DIR__, "..", "libsfmi35
 Run the O(N^2) pair histogram on one AMD MI355X (gfx950).  `precision = :fp64` is the drop-in
 binary64 kernel (counts bit-identical to SerialBackend); `precision = :mixed` runs the pair
 arithmetic in binary32 with binary64 re-binning of pairs near bin edges (counts still
-bit-identical) and binary64 accumulation.
+bit-identical) and binary64 accumulation; `precision = :mixed_fixed` additionally keeps joint
+cells of uniform value bins as 64-bit fixed-point words on the device (each value's extra
+absolute error at most 2^-23 of its value bin's width).
 """
 struct MI355XBackend <: CB.AbstractExecutionBackend
     device::Int
@@ -87,6 +92,8 @@ _opcode(::SFT.ThirdOrderStructureFunctionType) = (Int32(2), Int32(0), Int32(0))
 _opcode(::SFT.TransverseComponentSecondOrderStructureFunctionType) = (Int32(3), Int32(0), Int32(0))
 _opcode(::SFT.LongitudinalTransverseComponentThirdOrderStructureFunctionType) = (Int32(4), Int32(0), Int32(0))
 _opcode(::SFT.FullVectorStructureFunctionType{NF}) where {NF} = (Int32(5), Int32(NF), Int32(0))
+struct _SinglePass end  # SF_OP_SINGLE_PASS: rows S2, L2, T2, S3, L3, L1T2 of a (6, n_bins) output
+_opcode(::_SinglePass) = (Int32(6), Int32(0), Int32(0))
 _opcode(sf) = throw(ArgumentError("$(typeof(sf)) is not supported by MI355XBackend"))
 
 # geometry codes (SF_GEOM_*), from pair_geometry_for (HelperFunctions.jl:390-393)
@@ -101,7 +108,7 @@ _edges(b::SF.InfPaddedBinEdges) = vcat(-Inf, collect(Float64, b.edges), Inf)
 _edges(b::AbstractVector) = collect(Float64, b)
 
 _count_bits(::Type{CT}) where {CT} = sizeof(CT) <= 4 ? Int32(32) : Int32(64)
-_prec(b::MI355XBackend) = b.precision === :mixed ? Int32(1) : Int32(0)
+_prec(b::MI355XBackend) = b.precision === :mixed ? Int32(1) : b.precision === :mixed_fixed ? Int32(2) : Int32(0)
 
 function _run!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
                dbins, vbins, metric, CT)
@@ -178,6 +185,45 @@ function SFC._dispatch_execution_backend!(
     sums .+= reshape(s, size(sums))
     counts .+= reshape(c, size(counts))
     return nothing
+end
+
+# ---- single pass: six invariants in one pair pass (serial_single_pass.jl:400-535) -------------
+# The library accumulates rows 1, 2, 4, 5 and count row 1, then rewrites rows 3, 6 and count rows
+# 2..6 over the whole buffer, exactly as _sp1d_derive_rows! (serial_single_pass.jl:208-219).
+function SFC._dispatch_single_pass(
+    b::MI355XBackend, ::SFC.PointField, x::AbstractMatrix, u::AbstractMatrix, distance_bins::AbstractVector;
+    count_eltype::Type{CT} = UInt32, distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {CT}
+    nb = length(distance_bins) - 1
+    sums = zeros(Float64, 6 * nb)
+    counts = zeros(UInt64, 6 * nb)
+    _run!(b, sums, counts, _SinglePass(), x, u, distance_bins, nothing, distance_metric, CT)
+    return (sums = reshape(sums, 6, nb), counts = CT.(reshape(counts, 6, nb)))
+end
+
+function SFC._dispatch_single_pass!(
+    b::MI355XBackend, sums::AbstractMatrix, counts::AbstractMatrix, x::AbstractMatrix, u::AbstractMatrix,
+    distance_bins::AbstractVector; distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+)
+    size(sums, 1) == 6 && size(counts, 1) == 6 ||
+        throw(DimensionMismatch("sums and counts must have shape (6, n_bins)"))
+    s = zeros(Float64, length(sums))
+    c = zeros(UInt64, length(counts))
+    _run!(b, s, c, _SinglePass(), x, u, distance_bins, nothing, distance_metric, eltype(counts))
+    s = reshape(s, size(sums))
+    c = reshape(c, size(counts))
+    @inbounds for bin in axes(sums, 2)
+        for r in (1, 2, 4, 5)
+            sums[r, bin] += s[r, bin]
+        end
+        counts[1, bin] += c[1, bin]
+        sums[3, bin] = sums[1, bin] - sums[2, bin]
+        sums[6, bin] = sums[4, bin] - sums[5, bin]
+        for r in 2:6
+            counts[r, bin] = counts[1, bin]
+        end
+    end
+    return sums, counts
 end
 
 end # module
