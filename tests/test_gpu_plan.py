@@ -67,3 +67,40 @@ def test_plan_graph_replays_match_eager(ctx, prec, joint):
                 assert np.array_equal(gc, c.ravel(order="F")), (k, n)
         float(junk)
     plan.close()
+
+
+def test_autotune_then_call_matches_fp64(ctx):
+    """sf_autotune picks a launch shape for a problem class and later calls with a matching key use
+    it; whatever it picks, counts stay equal to the binary64 kernel's."""
+    torch = pytest.importorskip("torch")
+    from sfamd import synthetic
+    x, u, de, ve = synthetic.config4(N=20000)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+    du = torch.from_numpy(np.ascontiguousarray(u.T)).to(dev)
+    nd, nv = len(de) - 1, len(ve) - 1
+
+    def prob(prec):
+        p = _native.SfProblem()
+        p.D, p.W, p.N = 2, 2, x.shape[1]
+        p.x, p.u = dx.data_ptr(), du.data_ptr()
+        p.geometry, p.radius = 1, 1.0
+        p.op, p.nl, p.nt = 0, 1, 0
+        p.n_dist, p.dist_edges = nd, de.ctypes.data
+        p.n_val, p.val_edges = nv, ve.ctypes.data
+        p.precision, p.count_bits = prec, 64
+        return p
+
+    block, jchunk, bpc, variant = ctx.autotune([prob(1)])
+    assert block in (256, 512, 768, 1024) and variant in (0, 1, 2) and jchunk >= 0
+    outs = {}
+    for prec in (0, 1):
+        s = torch.zeros(nd * nv, dtype=torch.float64, device=dev)
+        c = torch.zeros(nd * nv, dtype=torch.int64, device=dev)
+        ctx.hist_device([prob(prec)], s.data_ptr(), c.data_ptr(), 0)
+        torch.cuda.synchronize()
+        outs[prec] = (s.cpu().numpy(), c.cpu().numpy())
+    assert ctx.stats()["block"] == block
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ok = outs[0][1] > 0
+    assert np.all(np.abs(outs[1][0] - outs[0][0])[ok] <= 1e-5 * outs[0][1][ok] * 4.0)
