@@ -173,7 +173,8 @@ int sf_check(int32_t n_slices, const sf_problem *slices, char *msg, int32_t msg_
  * Host pointers; n values.  (Diagnostic; no reference counterpart.) */
 int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a, const double *b, double *out);
 
-/* Kernel tuning override (autotuner hook): threads per workgroup (= rows of an i-tile; 64..1024),
+/* Kernel tuning override (autotuner hook): threads per workgroup (= rows of an i-tile; a multiple
+ * of 64 in 64..1024),
  * column-range length per work item, workgroups per CU, and the column-record variant of the
  * binary32 kernel (1: staged through LDS, 2: scalar loads).  Zero keeps the built-in choice.
  * Applies to subsequent calls on this context.  (No reference counterpart; the reference's GPU

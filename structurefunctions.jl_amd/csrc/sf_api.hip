@@ -587,8 +587,8 @@ extern "C" int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a,
 extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu,
                              int32_t variant) {
   if (!ctx) return SF_E_ARG;
-  if (block != 0 && block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
-    return ctx->fail(SF_E_ARG, "block must be 0, 64, 128, 256, 512 or 1024");
+  if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
+    return ctx->fail(SF_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
   if (j_chunk < 0 || blocks_per_cu < 0) return ctx->fail(SF_E_ARG, "tuning values must be >= 0");
   if (variant < 0 || variant > 2) return ctx->fail(SF_E_ARG, "variant must be 0, 1 or 2");
   ctx->tune_variant = variant;
