@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -46,7 +47,7 @@ int hist_opk(const Params &P);
 struct PrepSliceHost {
   const double *x, *u;
   int64_t N, pt_off;
-  double xscale;
+  const int32_t *perm;
 };
 }  // namespace sfk
 
@@ -181,6 +182,8 @@ struct Plan {
   DevBuf<Pending> pending;
   DevBuf<PrepSliceHost> prep;
   DevBuf<int> flag;
+  DevBuf<int32_t> perm;   // spatial order of the binary32 records (tile culling)
+  bool culled = false;    // some work items were dropped by tile culling
   hipGraphExec_t graph = nullptr;
   double *g_sums = nullptr;
   unsigned long long *g_counts = nullptr;
@@ -191,6 +194,7 @@ struct Plan {
     rec64.release(); rec32.release(); slices.release(); items.release(); cells.release();
     dedges.release(); vedges.release(); vedges32.release(); acc_s.release(); acc_c.release();
     counter.release(); pcount.release(); pending.release(); prep.release(); flag.release();
+    perm.release();
   }
 };
 
@@ -600,9 +604,153 @@ extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_
 
 namespace {
 
+// ---- spatial sort and tile culling ----------------------------------------------------------
+// When every distance edge of a slice is finite, a pair farther apart than the last edge is not
+// counted.  Points are put in Morton (Z-curve) order of their key-space coordinates (flat: x;
+// sphere: the unit vector), so that a tile of consecutive points is spatially compact, and a
+// work item (row tile, column chunk) whose bounding boxes are farther apart than the last edge is
+// dropped.  Only the binary32 records are reordered (each keeps its caller index); the binary64
+// records, and with them every re-binned pair, stay in the caller's order.
+// fn(begin, end) over [0, n) split across host threads
+template <typename Fn>
+void parallel_range(int64_t n, Fn fn) {
+  const int nt = (int)std::min<int64_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+                                        std::max<int64_t>(1, n / 16384));
+  if (nt <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
+  for (auto &x : th) x.join();
+}
+
+struct Box {
+  double lo[3], hi[3];
+};
+
+uint32_t spread10(uint32_t v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+struct CullInfo {
+  bool active = false;
+  double limit = 0;               // a pair of boxes farther apart than this holds no counted pair
+  std::vector<int32_t> perm;      // record k is point perm[k]
+  std::vector<Box> tiles;         // bounding box of each `rows`-point tile, in record order
+};
+
+int cull_prepare(sf_ctx *ctx, const sf_problem &p, const double *dx, double xmax, int rows,
+                 hipStream_t st, CullInfo &ci) {
+  ci.active = false;
+  const int64_t N = p.N;
+  const double emax = p.dist_edges[p.n_dist];
+  if (N < 4 * (int64_t)rows || !(emax < INFINITY)) return SF_OK;
+  const bool flat0 = p.geometry == SF_GEOM_FLAT;
+  // largest distance in key-space units: flat |dx|; sphere chord 2 sin(theta / 2)
+  double reach;
+  if (flat0) {
+    reach = std::max(emax, 0.0);
+  } else {
+    const double R = p.geometry == SF_GEOM_SPHERE_DEG ? p.radius : 1.0;
+    const double th = std::min(std::max(emax, 0.0) / R, M_PI);
+    reach = 2.0 * sin(0.5 * th);
+  }
+  // cheap refusals before touching the data: the sphere's extent is at most 2, a flat slice's
+  // at most 2 max|x| (when the caller measured it)
+  if (!flat0 && reach >= 1.0) return SF_OK;
+  if (flat0 && xmax > 0 && reach >= xmax) return SF_OK;
+  std::vector<double> hx((size_t)p.W * N);
+  HIPCHK(hipMemcpyAsync(hx.data(), dx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const bool flat = p.geometry == SF_GEOM_FLAT;
+  std::vector<double> kc((size_t)3 * N, 0.0);
+  parallel_range(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      if (flat) {
+        for (int k = 0; k < p.D; ++k) kc[3 * i + k] = hx[(size_t)p.W * i + k];
+      } else {
+        const double f = p.geometry == SF_GEOM_SPHERE_DEG ? M_PI / 180.0 : 1.0;
+        const double lon = hx[2 * i] * f, lat = hx[2 * i + 1] * f;
+        kc[3 * i] = cos(lat) * cos(lon);
+        kc[3 * i + 1] = cos(lat) * sin(lon);
+        kc[3 * i + 2] = sin(lat);
+      }
+    }
+  });
+  double cmax = 1.0;
+  for (double v : kc) {
+    if (!(fabs(v) < INFINITY)) return SF_OK;  // non-finite coordinates: no culling
+    cmax = std::max(cmax, fabs(v));
+  }
+  // margin: relative 1e-6 plus the rounding of coordinates of magnitude cmax
+  ci.limit = reach * (1.0 + 1e-6) + 1e-13 * cmax;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = 0; i < N; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], kc[3 * i + k]);
+      hi[k] = std::max(hi[k], kc[3 * i + k]);
+    }
+  // worth it only if the reach is small against the extent
+  double ext = 0;
+  for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+  if (!(ci.limit < 0.5 * ext)) return SF_OK;
+  std::vector<uint32_t> key((size_t)N);
+  parallel_range(N, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k) {
+        const double w = hi[k] - lo[k];
+        const double t = w > 0 ? (kc[3 * i + k] - lo[k]) / w : 0.0;
+        q[k] = (uint32_t)std::min(1023.0, std::max(0.0, floor(t * 1024.0)));
+      }
+      key[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+    }
+  });
+  // stable LSD radix sort of the 30-bit keys, 10 bits per pass
+  ci.perm.resize((size_t)N);
+  std::vector<int32_t> tmp((size_t)N);
+  for (int64_t i = 0; i < N; ++i) ci.perm[i] = (int32_t)i;
+  for (int pass = 0; pass < 3; ++pass) {
+    const int sh = 10 * pass;
+    std::vector<int64_t> cnt(1025, 0);
+    for (int64_t k = 0; k < N; ++k) cnt[((key[ci.perm[k]] >> sh) & 1023u) + 1]++;
+    for (int b = 0; b < 1024; ++b) cnt[b + 1] += cnt[b];
+    for (int64_t k = 0; k < N; ++k) tmp[cnt[(key[ci.perm[k]] >> sh) & 1023u]++] = ci.perm[k];
+    ci.perm.swap(tmp);
+  }
+  const int64_t nt = (N + rows - 1) / rows;
+  ci.tiles.assign((size_t)nt, Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}});
+  for (int64_t k = 0; k < N; ++k) {
+    Box &b = ci.tiles[k / rows];
+    const int64_t i = ci.perm[k];
+    for (int c = 0; c < 3; ++c) {
+      b.lo[c] = std::min(b.lo[c], kc[3 * i + c]);
+      b.hi[c] = std::max(b.hi[c], kc[3 * i + c]);
+    }
+  }
+  ci.active = true;
+  return SF_OK;
+}
+
+// true if no point of box a is within `limit` of a point of box b
+bool boxes_apart(const Box &a, const Box &b, double limit) {
+  double g2 = 0;
+  for (int k = 0; k < 3; ++k) {
+    const double g = std::max(0.0, std::max(b.lo[k] - a.hi[k], a.lo[k] - b.hi[k]));
+    g2 += g * g;
+  }
+  return g2 > limit * limit;
+}
+
 // Host work and uploads for one batch.  x/u of each slice are device pointers.
 int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double *const *dx,
-               const double *const *du, const double *xmax, hipStream_t st) {
+               const double *const *du, const double *xmax, hipStream_t st, bool cull) {
   const sf_problem &p0 = sl[0];
   pl.ctx = ctx;
   pl.n = n;
@@ -629,15 +777,13 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     prep[s].u = du[s];
     prep[s].N = sl[s].N;
     prep[s].pt_off = pl.total_pts;
-    prep[s].xscale = 0;
+    prep[s].perm = nullptr;
     pl.total_pts += sl[s].N;
     pl.max_n = std::max<int64_t>(pl.max_n, sl[s].N);
     pl.total_pairs += pairs_of(sl[s].N);
   }
   HIPCHK(pl.rec64.ensure((size_t)pl.total_pts * S64 + 16));
   if (mixed) HIPCHK(pl.rec32.ensure((size_t)pl.total_pts * F32 + 16));
-  HIPCHK(pl.prep.ensure(n));
-  HIPCHK(hipMemcpyAsync(pl.prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
 
   // ---- tables
   std::vector<Slice> sdesc(n);
@@ -739,19 +885,52 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   // fixed-point cells: one item must stay below the flush limit (the LDS histogram is flushed
   // between items only)
   if (pl.fx) jchunk = std::max(1, std::min<int>(jchunk, (int)(kFxFlushLimit / block)));
+  // spatial sort and tile culling (binary32 kernel, one-shot calls): column chunks of one tile
+  std::vector<CullInfo> cinfo(n);
+  std::vector<int32_t> perm_all;
+  pl.culled = false;
+  if (cull && mixed) {
+    for (int s = 0; s < n; ++s) {
+      const int rc = cull_prepare(ctx, sl[s], dx[s], xmax[s], block, st, cinfo[s]);
+      if (rc) return rc;
+    }
+    bool any = false;
+    for (int s = 0; s < n; ++s) any = any || cinfo[s].active;
+    if (any) {
+      perm_all.assign((size_t)pl.total_pts, 0);
+      for (int s = 0; s < n; ++s) {
+        for (int64_t k = 0; k < sl[s].N; ++k)
+          perm_all[prep[s].pt_off + k] = cinfo[s].active ? cinfo[s].perm[k] : (int32_t)k;
+      }
+      HIPCHK(pl.perm.ensure(perm_all.size()));
+      HIPCHK(hipMemcpyAsync(pl.perm.p, perm_all.data(), sizeof(int32_t) * perm_all.size(),
+                            hipMemcpyHostToDevice, st));
+      for (int s = 0; s < n; ++s)
+        if (cinfo[s].active) prep[s].perm = pl.perm.p + prep[s].pt_off;
+    }
+  }
+  HIPCHK(pl.prep.ensure(n));
+  HIPCHK(hipMemcpyAsync(pl.prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
+
   std::vector<int> order(n);
   for (int s = 0; s < n; ++s) order[s] = s;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sl[a].N > sl[b].N; });
   std::vector<Item> items;
   for (int s : order) {
     const int64_t N = sl[s].N;
+    const CullInfo &ci = cinfo[s];
+    const int64_t jc = ci.active ? block : jchunk;
     for (int64_t i0 = 0; i0 < N - 1; i0 += block) {
-      for (int64_t j0 = i0; j0 < N; j0 += jchunk) {
+      for (int64_t j0 = i0; j0 < N; j0 += jc) {
+        if (ci.active && j0 > i0 && boxes_apart(ci.tiles[i0 / block], ci.tiles[j0 / block], ci.limit)) {
+          pl.culled = true;
+          continue;
+        }
         Item it;
         it.slice = s;
         it.i0 = (int32_t)i0;
         it.j0 = (int32_t)j0;
-        it.j1 = (int32_t)std::min<int64_t>(N, j0 + jchunk);
+        it.j1 = (int32_t)std::min<int64_t>(N, j0 + jc);
         items.push_back(it);
       }
     }
@@ -872,7 +1051,7 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
               const double *const *du, const double *xmax, double *out_s,
               unsigned long long *out_c, hipStream_t st) {
   Plan &pl = ctx->oneshot;
-  int rc = plan_build(ctx, pl, n, sl, dx, du, xmax, st);
+  int rc = plan_build(ctx, pl, n, sl, dx, du, xmax, st, true);
   if (rc) return rc;
   for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
   ctx->stats[0] = pl.total_pairs;
@@ -891,7 +1070,12 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats[1] = (int64_t)pc;
     if ((int64_t)pc > pl.cap) {
-      // the merge was skipped; redo everything with the binary64 kernel
+      // the merge was skipped; redo everything with the binary64 kernel (which evaluates every
+      // pair in the caller's order: rebuild the work items without culling first)
+      if (pl.culled) {
+        rc = plan_build(ctx, pl, n, sl, dx, du, xmax, st, false);
+        if (rc) return rc;
+      }
       rc = plan_launch(pl, false, out_s, out_c, st, ctx->ev);
       if (rc) return rc;
     }
@@ -1043,7 +1227,8 @@ extern "C" int sf_plan_create(sf_ctx *ctx, int32_t n, const sf_problem *sl, sf_p
   }
   sf_plan *plan = new sf_plan();
   plan->ctx = ctx;
-  rc = plan_build(ctx, plan->pl, n, sl, px.data(), pu.data(), xmax.data(), st);
+  // no culling in plans: the positions behind x may change between executions
+  rc = plan_build(ctx, plan->pl, n, sl, px.data(), pu.data(), xmax.data(), st, false);
   if (rc) {
     plan->pl.release();
     delete plan;

@@ -19,16 +19,20 @@ enum Geom : int { kFlat = 0, kSphere = 1 };
 //                 E = local east (E2 = 0), N = local north (HelperFunctions.jl:133-136)
 //   binary32 record (mixed path), every binary64 input carried as a hi + lo pair of floats
 //   (v = hi + lo to ~2^-48 relative) so that differences of nearby values stay accurate:
-//     flat   D=2: xh0 xh1 xl0 xl1 uh0 uh1 ul0 ul1 |u| 0 0 0                   (12 floats)
-//     flat   D=3: xh0..2 xl0..2 uh0..2 ul0..2 |u| 0 0 0                       (16 floats)
-//     sphere D  : ph0..2 pl0..2 Uh0..2 Ul0..2 |u| urh url 0                  (16 floats)
+//     flat   D=2: xh0 xh1 xl0 xl1 uh0 uh1 ul0 ul1 |u| id 0 0                  (12 floats)
+//     flat   D=3: xh0..2 xl0..2 uh0..2 ul0..2 |u| id 0 0                      (16 floats)
+//     sphere D  : ph0..2 pl0..2 Uh0..2 Ul0..2 |u| urh url id                 (16 floats)
 //                 U = u_e E + u_n N is the tangent velocity in R^3, ur the radial component
-//                 (D = 3), P the unit position; |u| is an upper bound of the velocity norm.
+//                 (D = 3), P the unit position; |u| is an upper bound of the velocity norm;
+//                 id is the point's index in the caller's order (int32 bits): binary32 records
+//                 may be stored in a spatially sorted order (tile culling).
 inline constexpr __host__ __device__ int rec_d64(int G, int D) { return (G == kFlat) ? ((D == 2) ? 4 : 6) : 12; }
 inline constexpr __host__ __device__ int rec_f32(int G, int D) { return (G == kFlat && D == 2) ? 12 : 16; }
+inline constexpr __host__ __device__ int rec_id(int G, int D) { return (G == kFlat) ? 4 * D + 1 : 15; }
 template <int G, int D> struct Rec {
   static constexpr int d64 = rec_d64(G, D);
   static constexpr int f32 = rec_f32(G, D);
+  static constexpr int id = rec_id(G, D);
 };
 
 // Distance-key cell table entry.  The positive key axis (flat: r^2; sphere: chord^2 for the

@@ -299,7 +299,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
 struct PrepSlice {
   const double *x, *u;
   int64_t N, pt_off;
-  double xscale;  // unused placeholder (kept for alignment)
+  const int32_t *perm;  // binary32 record k is point perm[k] of the slice (NULL: identity)
 };
 
 template <int G, int D>
@@ -354,7 +354,8 @@ __global__ void prep32(const PrepSlice *ps, const double *rec64, float *rec32) {
   if (i >= s.N) return;
   constexpr int S = Rec<G, D>::d64;
   constexpr int F = Rec<G, D>::f32;
-  const double *r = rec64 + (s.pt_off + i) * S;
+  const int64_t src = s.perm ? (int64_t)s.perm[i] : i;
+  const double *r = rec64 + (s.pt_off + src) * S;
   float *o = rec32 + (s.pt_off + i) * F;
   for (int k = 0; k < F; ++k) o[k] = 0.0f;
   if (G == kFlat) {
@@ -377,6 +378,7 @@ __global__ void prep32(const PrepSlice *ps, const double *rec64, float *rec32) {
     o[12] = round_up_f(__builtin_sqrt(n2) * (1.0 + 1e-12));
     split2(r[10], &o[13], &o[14]);
   }
+  o[Rec<G, D>::id] = __int_as_float((int)src);
 }
 
 // =============================================================================================
@@ -892,7 +894,11 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     base = __shfl(base, leader);
     if (defer) {
       const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-      if (idx < (unsigned long long)P.pending_cap) P.pending[idx] = Pending{slice_id, i, j, 0};
+      // queued in the caller's order (records may be spatially sorted): the binary64 re-bin then
+      // evaluates the pair as the reference does, with i < j
+      const int oi = __float_as_int(ri[Rec<G, D>::id]), oj = __float_as_int(rj[Rec<G, D>::id]);
+      if (idx < (unsigned long long)P.pending_cap)
+        P.pending[idx] = Pending{slice_id, oi < oj ? oi : oj, oi < oj ? oj : oi, 0};
     }
   }
   int h;
