@@ -28,8 +28,10 @@ for p in (PKG, os.path.join(ROOT, "tests")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-# FLOP-equivalents per pair (FMA = 2, add/mul/cmp/cvt = 1, binary32 transcendental = 4) of the
-# algorithmic pair computation, per workload (derivation in DESIGN.md)
+# FLOP-equivalents per pair (FMA = 2, add/mul/cmp/cvt = 1, binary32 transcendental = 4; integer
+# and LDS work 0) of the algorithmic pair computation, per workload.  Spherical joint du_L: the
+# common spherical part 43 (differences, chord^2, chord^2-space bin, sin^2 sigma, ok test, rsq,
+# tangent projections) + du_L 8 + linear value bin 8 + one accumulation 1.
 FLOPS_PER_PAIR = {"sphere_joint_L1": 60, "flat2_L2": 28}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6

@@ -48,6 +48,7 @@ struct PrepSliceHost {
   const double *x, *u;
   int64_t N, pt_off;
   const int32_t *perm;
+  int64_t mul;
 };
 }  // namespace sfk
 
@@ -748,6 +749,27 @@ bool boxes_apart(const Box &a, const Box &b, double limit) {
   return g2 > limit * limit;
 }
 
+// Stride of the binary32 record order: record k holds point (k * mul) mod N.  Points that the
+// caller stores next to each other (a drifter track, a grid row) tend to fall in the same
+// (distance, value) cell against a common column point, so a wave whose 64 rows are neighbours
+// sends most of its LDS atomics to one or two cells and they serialize.  A stride near N times
+// the golden ratio conjugate (a Kronecker sequence) hands each wave rows spread over the whole
+// slice.  The pair set and the cells are unchanged; only the order of the additions differs.
+int64_t interleave_mul(int64_t N) {
+  if (N < 128) return 0;
+  int64_t m = (int64_t)((double)N * 0.6180339887498949);
+  auto gcd = [](int64_t a, int64_t b) {
+    while (b) {
+      const int64_t t = a % b;
+      a = b;
+      b = t;
+    }
+    return a;
+  };
+  while (gcd(m, N) != 1) ++m;
+  return m;
+}
+
 // Host work and uploads for one batch.  x/u of each slice are device pointers.
 int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double *const *dx,
                const double *const *du, const double *xmax, hipStream_t st, bool cull) {
@@ -778,11 +800,13 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     prep[s].N = sl[s].N;
     prep[s].pt_off = pl.total_pts;
     prep[s].perm = nullptr;
+    prep[s].mul = mixed ? interleave_mul(sl[s].N) : 0;
     pl.total_pts += sl[s].N;
     pl.max_n = std::max<int64_t>(pl.max_n, sl[s].N);
     pl.total_pairs += pairs_of(sl[s].N);
   }
   HIPCHK(pl.rec64.ensure((size_t)pl.total_pts * S64 + 16));
+  // + one record (F32 <= 16): the binary32 pair loop requests the record after its last column
   if (mixed) HIPCHK(pl.rec32.ensure((size_t)pl.total_pts * F32 + 16));
 
   // ---- tables
@@ -853,21 +877,32 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   int copies = 1;
   if (!J) {
     copies = 64;
-    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0).total * 2 > ctx->max_smem)
+    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0).total * 2 > ctx->max_smem)
       copies >>= 1;
   }
-  const SmemLayout L = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx);
-  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom, 0);
-  if ((size_t)std::max(L.total, L64.total) > ctx->max_smem)
+  const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0);
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom, 0, 0);
+  if ((size_t)std::max(L0.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
-                     ncell, std::max(L.total, L64.total), ctx->max_smem);
-  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L.total);
+                     ncell, std::max(L0.total, L64.total), ctx->max_smem);
+  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L0.total);
   if (bpc < 1) bpc = 1;
   if (!t_block && pl.fx && bpc * block < 1024) {
     // registers allow only one 768-thread workgroup per CU: use 1024 threads instead
     block = 1024;
-    bpc = std::max(1, hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L.total));
+    bpc = std::max(1, hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L0.total));
+  }
+  // binary32 kernel: the LDS re-bin queue, as large as fits without costing a workgroup per CU
+  // (0: every queued pair takes the global queue's atomic)
+  int qcap = 0;
+  if (mixed) {
+    for (int q = kQueueCap; q >= 64 && qcap == 0; q >>= 1) {
+      const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q).total;
+      if ((size_t)t <= ctx->max_smem &&
+          hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)t) >= bpc)
+        qcap = q;
+    }
   }
   if (t_bpc) bpc = std::min(bpc, t_bpc);
   const int max_blocks = bpc * ctx->n_cu;
@@ -1005,6 +1040,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   P.vcoef = (float)ldexp(16.0 * std::max(1, P.order), -24);
   P.nmom = nmom;
   P.fx = pl.fx ? 1 : 0;
+  P.qcap = qcap;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
   return SF_OK;
 }
@@ -1028,11 +1064,12 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
   Params P = pl.P;
   P.stage_bytes = stage;
-  if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts
+  if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts, no queue
     P.fx = 0;
+    P.qcap = 0;
     P.flush_limit = 0x7fffffffLL;
   }
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom, P.fx);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom, P.fx, P.qcap);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));

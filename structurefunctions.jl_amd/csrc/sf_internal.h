@@ -108,7 +108,7 @@ struct Params {
   float vcoef;             // binary32 value error coefficient: |dv| <= vcoef * Umax^order * amp
   int32_t nmom;            // accumulated moments per bin: 1, or 4 for the single-pass invariants
   int32_t fx;              // joint histogram kept as 64-bit fixed-point words (kFx* below)
-  int32_t pad2;
+  int32_t qcap;            // binary32 kernel: re-bin queue entries staged in LDS per workgroup
   int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
 };
 
@@ -127,7 +127,7 @@ inline constexpr __host__ __device__ int sp_row(int k) { return k < 2 ? k : k + 
 
 // LDS layout of the histogram kernels, shared by the host (sizing) and the device (carving).
 struct SmemLayout {
-  int32_t hs, hc, tab, dE, vE, vE32, stage, total;  // byte offsets
+  int32_t hs, hc, tab, dE, vE, vE32, stage, q, total;  // byte offsets
 };
 
 // LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each): joint histograms are stored value-bin fastest with an odd row stride per
@@ -135,9 +135,13 @@ struct SmemLayout {
 // atomics then fall in different banks); 1D histograms keep one copy per lane group.
 inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
 inline constexpr int kDummySlots = 64;
+// largest re-bin queue staged in LDS per workgroup (entries); the host shrinks it where the LDS
+// it takes would cost a workgroup per CU, and entries past it go straight to the global queue
+inline constexpr int kQueueCap = 1024;
 
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
-                                                  int stage_bytes, int copies, int nmom, int fx) {
+                                                  int stage_bytes, int copies, int nmom, int fx,
+                                                  int qcap) {
   SmemLayout L;
   // + kDummySlots: one scratch slot per lane, target of unconditional atomics of lanes that have
   // nothing to add (never flushed)
@@ -150,6 +154,8 @@ inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int nce
   L.vE = o; o += 8 * ((n_val + 2) & ~1);
   L.vE32 = o; o += 4 * ((n_val + 4) & ~3);
   L.stage = o; o += stage_bytes;
+  // re-bin queue: a 16-byte counter word, then qcap (i, j) entries of 8 bytes
+  L.q = o; o += qcap > 0 ? 16 + 8 * qcap : 0;
   L.total = o;
   return L;
 }

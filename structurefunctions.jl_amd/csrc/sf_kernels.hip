@@ -299,7 +299,8 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
 struct PrepSlice {
   const double *x, *u;
   int64_t N, pt_off;
-  const int32_t *perm;  // binary32 record k is point perm[k] of the slice (NULL: identity)
+  const int32_t *perm;  // binary32 record k is point perm[k] of the slice (NULL: see mul)
+  int64_t mul;          // without perm: record k is point (k * mul) mod N (0: identity)
 };
 
 template <int G, int D>
@@ -354,7 +355,7 @@ __global__ void prep32(const PrepSlice *ps, const double *rec64, float *rec32) {
   if (i >= s.N) return;
   constexpr int S = Rec<G, D>::d64;
   constexpr int F = Rec<G, D>::f32;
-  const int64_t src = s.perm ? (int64_t)s.perm[i] : i;
+  const int64_t src = s.perm ? (int64_t)s.perm[i] : (s.mul ? (i * s.mul) % s.N : i);
   const double *r = rec64 + (s.pt_off + src) * S;
   float *o = rec32 + (s.pt_off + i) * F;
   for (int k = 0; k < F; ++k) o[k] = 0.0f;
@@ -402,10 +403,14 @@ struct Smem {
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
   bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
   unsigned long long *hq;
+  int qcap;                 // re-bin queue staged in LDS: capacity, fill counter, (i, j) entries
+  unsigned *qn;
+  int2 *q;
 };
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx);
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx,
+                                   P.qcap);
   Smem m;
   m.nmom = P.nmom;
   m.fx = P.fx != 0;
@@ -426,6 +431,9 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.vE = (double *)(base + L.vE);
   m.vE32 = (float *)(base + L.vE32);
   m.stage = base + L.stage;
+  m.qcap = P.qcap;
+  m.qn = (unsigned *)(base + L.q);
+  m.q = (int2 *)(base + L.q + 16);
   return m;
 }
 
@@ -582,11 +590,13 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
   constexpr int S = Rec<G, D>::d64;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
+  __shared__ unsigned long long s_qbase;
   const int B = blockDim.x;
   int cur = -1;
   Slice s{};
   unsigned long long since_flush = 0;
   zero_hist(m);
+  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -740,14 +750,14 @@ struct PendingAdd {
 };
 
 // One pair (i, j): certified distance bin from the binary32 key, operator value, certified value
-// bin, queueing of undecidable pairs, and the deferred update.  rj is column j's record; the
-// record of column j + 1 (if j + 1 < j_end) is requested into rn.  The body has no branch on the
-// pair's own state: every pair is evaluated (out-of-range pairs are simply not counted), so the
-// wave never splits.
+// bin, and the deferred update.  rj is column j's record; the record of column j + 1 (if
+// j + 1 < j_end) is requested into rn.  The body has no branch on the pair's own state: every
+// pair is evaluated (out-of-range pairs are simply not counted), so the wave never splits.
+// Returns whether the pair is undecidable in binary32 and must be re-binned in binary64; the
+// caller collects these flags and queues the pairs every few columns (queue_deferred).
 template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
-DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i, int j,
-                const float *rj, float *rn, const float4 *next, bool has_next, int slice_id,
-                PendingAdd &pa) {
+DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i, int j,
+                const float *rj, float *rn, const float4 *next, bool has_next, PendingAdd &pa) {
   constexpr int F = Rec<G, D>::f32;
   // OPK 3 (single pass) needs du_L and |du|^2; on the sphere |du|^2 = du_L^2 + |du_T|^2
   constexpr bool SP = (OPK == 3);
@@ -791,11 +801,14 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     amb_d = (key >= lh.x) & (key <= lh.y);
     db = (key > lh.y) ? blo + 1 : blo;
   }
-  if (has_next) {
+  // scalar-load variant: the request is unconditional (past the last column it reads the record
+  // array's one-record tail pad) and pinned here, so that the scheduler does not sink it towards
+  // its use and this pair's arithmetic covers its latency
+  if constexpr (SL) {
     load_rec<F>(next, 0, rn);
-    // scalar-load variant: one 64-byte request per 16-float record rather than a split 8+4+1
-    // (the scalar data cache is request-bound here)
-    if constexpr (SL && F == 16) asm volatile("" ::"s"(rn[13]), "s"(rn[14]), "s"(rn[15]));
+    __builtin_amdgcn_sched_barrier(0);
+  } else if (has_next) {
+    load_rec<F>(next, 0, rn);
   }
   amb = amb_d;
   const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
@@ -885,33 +898,71 @@ DEV void pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   // binned here: certain and in range; queued: undecidable and not certainly out of range
   const bool add = upper & !amb & db_ok & vb_ok;
   const bool defer = upper & amb & (amb_d | db_ok);
-  const unsigned long long mask = __ballot(defer);
-  if (mask) {
-    const int lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(P.pending_count, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    if (defer) {
-      const unsigned long long idx = base + __popcll(mask & ((1ull << lane) - 1ull));
-      // queued in the caller's order (records may be spatially sorted): the binary64 re-bin then
-      // evaluates the pair as the reference does, with i < j
-      const int oi = __float_as_int(ri[Rec<G, D>::id]), oj = __float_as_int(rj[Rec<G, D>::id]);
-      if (idx < (unsigned long long)P.pending_cap)
-        P.pending[idx] = Pending{slice_id, oi < oj ? oi : oj, oi < oj ? oj : oi, 0};
-    }
-  }
   int h;
   if constexpr (J) h = vb0 + m.vstride * (db - 1);
   else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
   pa.h = add ? h : -1;
   pa.v = SP ? L : v;
   pa.w = n2;
+  // scalar-load variant: an empty asm that reads the last three words of the next record makes
+  // the compiler fetch the 16-float record in one 64-byte request rather than a split 8+4+1 (the
+  // scalar data cache is request-bound here).  It sits at the end of the pair so that the wait
+  // it implies comes after this pair's arithmetic, not right after the request.
+  if constexpr (SL && F == 16) asm volatile("" ::"s"(rn[13]), "s"(rn[14]), "s"(rn[15]));
+  return defer;
 }
 
-// Pairs (i, j) for j in [j_begin, j_end); column j's record is col[(j - col_j0) * F/4 ...],
-// either the LDS stage or (scalar-load variant) the record array itself.  Two columns per step
-// with alternating record buffers, so that the prefetched record never has to be copied.
+// Queue the pairs (i, jg + t) whose bit t is set in dm for the binary64 re-bin.  Kept out of the
+// pair loop: a branch there (with its global atomic and lane exchange) splits the loop body into
+// pieces the compiler cannot overlap.  Entries are staged in the workgroup's LDS queue and moved
+// to the global queue once per work item (queue_flush): a returning global atomic on the one
+// queue counter, shared by every wave of the device, costs thousands of cycles under that
+// contention.  Entries past the LDS capacity go to the global queue directly.  Pairs are queued
+// in the caller's order (records may be permuted or spatially sorted), so the binary64 re-bin
+// evaluates the pair as the reference does, with i < j.
+template <int G, int D>
+DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, const float *ri,
+                        const float4 *col, int col_j0, int jg, int slice_id) {
+  constexpr int R = Rec<G, D>::f32 / 4;
+  if (dm == 0u) return;
+  unsigned k = m.qcap > 0 ? atomicAdd(m.qn, (unsigned)__popc(dm)) : 0u;
+  const int oi = __float_as_int(ri[Rec<G, D>::id]);
+  while (dm) {
+    const int t = __ffs((int)dm) - 1;
+    dm &= dm - 1u;
+    const float *rj = (const float *)(col + (int64_t)(jg + t - col_j0) * R);
+    const int oj = __float_as_int(rj[Rec<G, D>::id]);
+    const int a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
+    if (k < (unsigned)m.qcap) {
+      m.q[k] = make_int2(a, b);
+    } else {
+      const unsigned long long g = atomicAdd(P.pending_count, 1ull);
+      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, a, b, 0};
+    }
+    ++k;
+  }
+}
+
+// Move the workgroup's LDS queue to the global queue (all threads; between barriers).
+DEV void queue_flush(const Params &P, const Smem &m, int slice_id, unsigned long long *s_base) {
+  if (m.qcap == 0) return;
+  const unsigned n0 = *m.qn;
+  const unsigned n = n0 < (unsigned)m.qcap ? n0 : (unsigned)m.qcap;
+  if (n == 0u) return;  // uniform: every thread read the same counter after the barrier
+  if (threadIdx.x == 0) *s_base = atomicAdd(P.pending_count, (unsigned long long)n);
+  __syncthreads();
+  const unsigned long long base = *s_base;
+  for (unsigned k = threadIdx.x; k < n; k += blockDim.x) {
+    const unsigned long long idx = base + k;
+    if (idx < (unsigned long long)P.pending_cap) {
+      const int2 e = m.q[k];
+      P.pending[idx] = Pending{slice_id, e.x, e.y, 0};
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *m.qn = 0u;
+}
+
 template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
 DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const float *ri, int i,
                         int j_begin, int j_end, const float4 *col, int col_j0, int slice_id) {
@@ -924,11 +975,25 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
   load_rec<F>(cp, 0, ra);
   PendingAdd pa{-1, 0.f, 0.f, 0u};
   int j = j_begin;
-  for (; j + 1 < j_end; j += 2, cp += 2 * R) {
-    pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, true, slice_id, pa);
-    pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j + 1, rb, ra, cp + 2 * R, j + 2 < j_end, slice_id, pa);
+  // groups of 32 columns: bit t of dm flags pair (i, jg + t) for the re-bin queue
+  while (j < j_end) {
+    const int jg = j;
+    const int je = min(j_end, jg + 32);
+    unsigned dm = 0u;
+    for (; j + 1 < je; j += 2, cp += 2 * R) {
+      const unsigned b = 1u << (j - jg);
+      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, true, pa)) dm |= b;
+      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j + 1, rb, ra, cp + 2 * R, j + 2 < j_end, pa))
+        dm |= b << 1;
+    }
+    // an odd count only occurs in the last group (the others hold 32 columns)
+    if (j < je) {
+      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, false, pa))
+        dm |= 1u << (j - jg);
+      ++j;
+    }
+    queue_deferred<G, D>(P, m, dm, ri, col, col_j0, jg, slice_id);
   }
-  if (j < j_end) pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, false, slice_id, pa);
   if (pa.h >= 0) {
     if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
     else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
@@ -955,11 +1020,13 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
+  __shared__ unsigned long long s_qbase;
   const int B = blockDim.x;
   int cur = -1;
   Slice s{};
   unsigned long long since_flush = 0;
   zero_hist(m);
+  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -1014,6 +1081,8 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
         }
       }
     }
+    __syncthreads();
+    queue_flush(P, m, cur, &s_qbase);
   }
   __syncthreads();
   if (cur >= 0) flush_hist(m, P, s);
