@@ -52,14 +52,20 @@ extern "C" {
 #define SF_OP_T2COMP 3        /* TransverseComponentSecondOrder... ||du_t||^2/(D-1) (:137,147)    */
 #define SF_OP_L1T2COMP 4      /* LongitudinalTransverseComponentThirdOrder...     (:145,150)      */
 #define SF_OP_FULLVEC 5       /* FullVectorStructureFunctionType{nl} ||du||^nl, nl in 1..3 (:114) */
-#define SF_OP_SINGLE_PASS 6   /* the six invariants S2, L2, T2, S3, L3, L1T2 in one pass
-                                 (calculate_structure_functions_single_pass[!],
-                                 src/Calculations/serial_single_pass.jl:404-421, 602-700).  nl, nt
-                                 ignored; n_val must be 0.  Output per slice is (6, n_dist)
-                                 column-major, rows in that order.  Rows 1, 2, 4, 5 and count row 1
-                                 accumulate; then, over the whole output, row 3 = row 1 - row 2,
-                                 row 6 = row 4 - row 5 and count rows 2..6 = count row 1
-                                 (_sp1d_derive_rows!, :193-219).                                 */
+#define SF_OP_SINGLE_PASS 6   /* the six invariants S2, L2, T2, S3, L3, L1T2 in one pass; nl, nt
+                                 ignored.
+                                 n_val = 0 (calculate_structure_functions_single_pass[!],
+                                 src/Calculations/serial_single_pass.jl:404-421, 602-700): output
+                                 per slice (6, n_dist) column-major, rows in that order.  Rows 1,
+                                 2, 4, 5 and count row 1 accumulate; then, over the whole output,
+                                 row 3 = row 1 - row 2, row 6 = row 4 - row 5 and count rows 2..6 =
+                                 count row 1 (_sp1d_derive_rows!, :193-219).
+                                 n_val > 0, the 2D form (calculate_structure_functions_single_pass_2d[!],
+                                 :712-728, 1161-1181): each invariant binned by distance and by its
+                                 own value edges; val_edges holds six edge vectors back to back
+                                 (6 * (n_val + 1) doubles, invariant t's at t * (n_val + 1)); output
+                                 per slice (6, n_dist, n_val) column-major, every cell accumulated.
+                                 Runs as six joint histograms, one per invariant.                 */
 
 /* precision */
 #define SF_PREC_FP64 0        /* plain binary64 pair arithmetic: the drop-in (counts bit-exact)  */
@@ -87,7 +93,8 @@ typedef struct {
                                  bins are (e[k-1], e[k]]                                          */
   const double *dist_edges;   /* host pointer                                                     */
   int32_t n_val;              /* 0 => 1D histogram; else value bins of the joint histogram        */
-  const double *val_edges;    /* host pointer, n_val + 1 ascending entries (or NULL)              */
+  const double *val_edges;    /* host pointer, n_val + 1 ascending entries (or NULL); six vectors
+                                 back to back for the 2D single pass (SF_OP_SINGLE_PASS)          */
   int32_t precision;          /* SF_PREC_*                                                        */
   int32_t count_bits;         /* 32 or 64: width of the caller's count type (UInt32 default,
                                  dispatch.jl:69); calls whose N(N-1)/2 exceeds it are rejected    */
@@ -110,7 +117,7 @@ int sf_hist(sf_ctx *ctx, const sf_problem *p, double *sums, uint64_t *counts);
 /* Many independent point fields ("slices", ragged N) in one persistent launch.  All slices must
  * share D, W, geometry, radius, operator, n_dist, n_val and precision; each slice has its own
  * edges.  Slice s accumulates into sums/counts[s * ncell .. (s+1) * ncell), ncell = n_dist * max(n_val,1)
- * (6 * n_dist for SF_OP_SINGLE_PASS).
+ * (6 * n_dist * max(n_val, 1) for SF_OP_SINGLE_PASS).
  * Replaces gpu_calculate_structure_function_batch[!] (gpu_stubs.jl:191-213), generalised from
  * the reference's equal-N batches (batch_leading.jl:43-63) to ragged N. */
 int sf_hist_batch(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *sums,

@@ -567,3 +567,132 @@ extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t 
   }
   return 0;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Single pass, 2D (joint) form: the six invariants each binned by (distance, own value)
+// (calculate_structure_functions_single_pass_2d, src/Calculations/serial_single_pass.jl:1161-1181;
+// pair loops _sp2d_simd_pairs! :873-916 for flat, _sp2d_pairs! :820-850 for the sphere).  Per
+// counted pair: du_L and |du|^2 as in the 1D form, du_L2 = du_L^2, du_T2 = |du|^2 - du_L2, values
+// (|du|^2, du_L2, du_T2, du_L |du|^2, du_L du_L2, du_L du_T2); value t goes to bin
+// digitize(value, vb_t) when 1 <= bin <= n_val (_sp2d_scatter!, :852-864).  val_edges6 holds the
+// six edge vectors back to back, n_val + 1 each.  Output (6, n_dist, n_val) column-major.
+// ---------------------------------------------------------------------------------------------
+extern "C" int sfo_hist_single_pass_2d(const sfo_problem *p, int nthreads, int64_t i_begin,
+                                       int64_t i_end, int64_t i_step, const double *val_edges6,
+                                       double *sums, uint64_t *counts, double *abs_sums,
+                                       double *mag_sums) {
+  if (!p || p->n_val < 1 || !val_edges6) return -10;
+  sfo_problem q = *p;
+  const int nv = q.n_val;
+  q.op = SFO_OP_S2;
+  q.nl = q.nt = 0;
+  q.n_val = 0;
+  q.val_edges = nullptr;
+  const int rc = validate(&q);
+  if (rc) return rc;
+  for (int t = 0; t < 6; ++t) {
+    sfo_problem qt = *p;
+    qt.op = SFO_OP_S2;
+    qt.nl = qt.nt = 0;
+    qt.val_edges = val_edges6 + (size_t)t * (nv + 1);
+    const int rt = validate(&qt);
+    if (rt) return rt;
+  }
+  if (!sums || !counts || i_step < 1) return -10;
+  if (i_begin < 0) i_begin = 0;
+  if (i_end > q.N) i_end = q.N;
+  Prob P;
+  P.p = &q;
+  prepare(P);
+  const int nb = q.n_dist, D = q.D, W = q.W;
+  const int64_t ncell = (int64_t)6 * nb * nv;
+  if (nthreads < 1) nthreads = 1;
+  struct Acc { std::vector<double> s, c, a, g; std::vector<uint64_t> n; };
+  std::vector<Acc> acc(nthreads);
+  for (auto &a : acc) {
+    a.s.assign(ncell, 0.0); a.c.assign(ncell, 0.0); a.n.assign(ncell, 0);
+    a.a.assign(ncell, 0.0); a.g.assign(ncell, 0.0);
+  }
+  auto work = [&](int th) {
+    Acc &a = acc[th];
+    int64_t k = 0;
+    for (int64_t i = i_begin; i < i_end; i += i_step, ++k) {
+      if ((int)(k % nthreads) != th) continue;
+      for (int64_t j = i + 1; j < q.N; ++j) {
+        double duL, dn2, scale;
+        int64_t b;
+        if (q.geometry == SFO_GEOM_FLAT) {
+          double dx[3] = {0, 0, 0}, du[3] = {0, 0, 0};
+          for (int d = 0; d < D; ++d) {
+            dx[d] = q.x[d + W * j] - q.x[d + W * i];
+            du[d] = q.u[d + D * j] - q.u[d + D * i];
+          }
+          const double r2 = sdot(dx, dx, D);
+          const double inv_r = 1.0 / sqrt(r2);
+          duL = sdot(du, dx, D) * inv_r;
+          dn2 = sdot(du, du, D);
+          scale = sqrt(dn2);
+          b = count_isless(P.sqedges.data(), nb + 1, r2);
+          if (!(1 <= b && b <= nb)) continue;
+        } else {
+          const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],
+                                            &P.E[3 * j], &P.Nn[3 * j], q.radius);
+          b = count_isless(q.dist_edges, nb + 1, f.distance);
+          if (!(f.ok && 1 <= b && b <= nb)) continue;
+          double du[3];
+          sphere_delta(f, &q.u[D * i], &q.u[D * j], D, du);
+          const double rh[3] = {1.0, 0.0, 0.0};
+          duL = sdot(du, rh, D);
+          dn2 = sdot(du, du, D);
+          // the frame transport makes binary32 errors absolute in the velocities' size, growing
+          // like 1 + c^2 / sin(sigma) towards antipodal pairs
+          const double sig = f.distance / q.radius, c2 = 4.0 * sin(0.5 * sig) * sin(0.5 * sig);
+          scale = (sqrt(sdot(&q.u[D * i], &q.u[D * i], D)) + sqrt(sdot(&q.u[D * j], &q.u[D * j], D))) *
+                  (1.0 + c2 / fmax(sin(sig), 1e-300));
+        }
+        const double duL2 = duL * duL;
+        const double duT2 = dn2 - duL2;
+        const double vals[6] = {dn2, duL2, duT2, duL * dn2, duL * duL2, duL * duT2};
+        for (int t = 0; t < 6; ++t) {
+          const int64_t vb = count_isless(val_edges6 + (size_t)t * (nv + 1), nv + 1, vals[t]);
+          if (!(1 <= vb && vb <= nv)) continue;
+          const int64_t cell = t + 6 * ((b - 1) + (int64_t)nb * (vb - 1));
+          double &sm = a.s[cell], &cm = a.c[cell];
+          const double v = vals[t], tt = sm + v;
+          if (fabs(sm) >= fabs(v)) cm += (sm - tt) + v; else cm += (v - tt) + sm;
+          sm = tt;
+          a.n[cell] += 1;
+          a.a[cell] += fabs(v);
+          a.g[cell] += t < 3 ? scale * scale : scale * scale * scale;  // the error scale^order
+        }
+      }
+    }
+  };
+  if (nthreads == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(work, t);
+    for (auto &x : th) x.join();
+  }
+  for (int64_t c = 0; c < ncell; ++c) {
+    double sm = 0.0, cm = 0.0;
+    uint64_t n = 0;
+    for (int t = 0; t < nthreads; ++t) {
+      const double parts[2] = {acc[t].s[c], acc[t].c[c]};
+      for (double v : parts) {
+        const double tt = sm + v;
+        if (fabs(sm) >= fabs(v)) cm += (sm - tt) + v; else cm += (v - tt) + sm;
+        sm = tt;
+      }
+      n += acc[t].n[c];
+    }
+    sums[c] += sm + cm;
+    counts[c] += n;
+    for (int t = 0; t < nthreads; ++t) {
+      if (abs_sums) abs_sums[c] += acc[t].a[c];
+      if (mag_sums) mag_sums[c] += acc[t].g[c];
+    }
+  }
+  return 0;
+}

@@ -63,6 +63,19 @@ int sfo_hist(const sfo_problem *p, int nthreads, int compensated, int64_t i_begi
 int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t i_begin, int64_t i_end,
                          int64_t i_step, double *sums, uint64_t *counts);
 
+/* Single pass, 2D (joint) form (calculate_structure_functions_single_pass_2d,
+ * src/Calculations/serial_single_pass.jl:1161-1181): each counted pair's six invariants
+ * (|du|^2, du_L^2, |du|^2 - du_L^2, du_L |du|^2, du_L^3, du_L (|du|^2 - du_L^2)) are binned by
+ * distance and each by its own value edges.  p->n_val is the number of value bins; val_edges6
+ * holds the six edge vectors back to back (n_val + 1 each); p->val_edges is ignored.  sums and
+ * counts are (6, n_dist, n_val) column-major, accumulated into; abs_sums / mag_sums (may be NULL)
+ * receive sum |value| and sum m^order (order 2 for the first three invariants, 3 else) with m the
+ * magnitude scale of binary32 errors: ||du|| (flat), (||u_i|| + ||u_j||)(1 + c^2/sin(sigma)) on the
+ * sphere, whose frame transport makes errors absolute in the velocities' size. */
+int sfo_hist_single_pass_2d(const sfo_problem *p, int nthreads, int64_t i_begin, int64_t i_end,
+                            int64_t i_step, const double *val_edges6, double *sums,
+                            uint64_t *counts, double *abs_sums, double *mag_sums);
+
 /* deterministic elementary functions (structurefunctions.jl_amd/csrc/sfmath.h), for tests */
 void sfo_sincosd(double x, double *s, double *c);
 void sfo_sincos(double x, double *s, double *c);

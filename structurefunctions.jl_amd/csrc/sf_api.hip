@@ -165,6 +165,7 @@ namespace {
 struct Plan {
   sf_ctx *ctx = nullptr;
   int n = 0, G = 0, D = 0, geom_code = 0, ncell = 0, S64 = 0, F32 = 0;
+  int nout = 0;  // output slices (n is the internal slice count: six per slice for 2D single pass)
   bool J = false, mixed = false, sload = false, sp = false, fx = false;
   int block = 256, tj = 256, copies = 1, grid = 1, ncell_max = 1, n_items = 0;
   int64_t total_pts = 0, max_n = 0, total_pairs = 0, cap = 0;
@@ -274,10 +275,7 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
       if (p->nl < 1 || p->nl > 3)
         return ctx->fail(SF_E_UNSUPPORTED, "FullVectorStructureFunctionType{%d}: NF must be 1..3", p->nl);
       break;
-    case SF_OP_SINGLE_PASS:
-      if (p->n_val != 0)
-        return ctx->fail(SF_E_UNSUPPORTED,
-                         "the single-pass invariants are binned by distance only here (n_val must be 0)");
+    case SF_OP_SINGLE_PASS:  // n_val > 0: the 2D form, six edge vectors (checked below)
       break;
     default:
       return ctx->fail(SF_E_ARG, "unknown operator code %d (derived operators are not pairwise)", p->op);
@@ -299,11 +297,16 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
   if (p->n_val < 0) return ctx->fail(SF_E_ARG, "n_val must be >= 0");
   if (p->n_val > 0) {
     if (!p->val_edges) return ctx->fail(SF_E_ARG, "value_bins missing");
-    for (int k = 0; k <= p->n_val; ++k) {
-      const double e = p->val_edges[k];
-      if (e != e) return ctx->fail(SF_E_ARG, "value_bins contains NaN");
-      if (k > 0 && jl_isless(e, p->val_edges[k - 1]))
-        return ctx->fail(SF_E_ARG, "value_bins must be sorted ascending");
+    // the single pass's 2D form carries one edge vector per invariant, back to back
+    const int nvec = sp2d(p->op, p->n_val) ? kSpRows : 1;
+    for (int t = 0; t < nvec; ++t) {
+      const double *ve = p->val_edges + (size_t)t * (p->n_val + 1);
+      for (int k = 0; k <= p->n_val; ++k) {
+        const double e = ve[k];
+        if (e != e) return ctx->fail(SF_E_ARG, "value_bins contains NaN");
+        if (k > 0 && jl_isless(e, ve[k - 1]))
+          return ctx->fail(SF_E_ARG, "value_bins must be sorted ascending");
+      }
     }
   }
   if (p->precision != SF_PREC_FP64 && p->precision != SF_PREC_MIXED && p->precision != SF_PREC_MIXED_FIXED)
@@ -447,7 +450,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   s.v_uniform = 0;
   s.v_negzero = 0;
   s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0;
-  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.pad0 = 0;
+  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0;
   if (p.n_val > 0) {
     const int nv = p.n_val;
     const double *E = p.val_edges;
@@ -483,7 +486,7 @@ int64_t pairs_of(int64_t n) { return n > 1 ? n * (n - 1) / 2 : 0; }
 
 // output cells per slice: (n_dist), (n_dist, n_val), or (6, n_dist) for the single pass
 int out_cells(const sf_problem &p) {
-  if (p.op == SF_OP_SINGLE_PASS) return kSpRows * p.n_dist;
+  if (p.op == SF_OP_SINGLE_PASS) return kSpRows * p.n_dist * (p.n_val > 0 ? p.n_val : 1);
   return p.n_dist * (p.n_val > 0 ? p.n_val : 1);
 }
 
@@ -771,19 +774,48 @@ int64_t interleave_mul(int64_t N) {
 }
 
 // Host work and uploads for one batch.  x/u of each slice are device pointers.
-int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double *const *dx,
-               const double *const *du, const double *xmax, hipStream_t st, bool cull) {
+int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const double *const *dx_in,
+               const double *const *du_in, const double *xmax_in, hipStream_t st, bool cull) {
+  // the single pass's 2D form runs every input slice as six plane slices (one joint histogram per
+  // invariant, its own value edges); plane t of input slice s writes output cells t, t + 6, ...
+  const bool planes = sp2d(sl_in[0].op, sl_in[0].n_val);
+  std::vector<sf_problem> ex_sl;
+  std::vector<const double *> ex_x, ex_u;
+  std::vector<double> ex_m;
+  int n = n_in;
+  const sf_problem *sl = sl_in;
+  const double *const *dx = dx_in;
+  const double *const *du = du_in;
+  const double *xmax = xmax_in;
+  if (planes) {
+    for (int s = 0; s < n_in; ++s) {
+      for (int t = 0; t < kSpRows; ++t) {
+        sf_problem q = sl_in[s];
+        q.val_edges = sl_in[s].val_edges + (size_t)t * (sl_in[s].n_val + 1);
+        ex_sl.push_back(q);
+        ex_x.push_back(dx_in[s]);
+        ex_u.push_back(du_in[s]);
+        ex_m.push_back(xmax_in[s]);
+      }
+    }
+    n = kSpRows * n_in;
+    sl = ex_sl.data();
+    dx = ex_x.data();
+    du = ex_u.data();
+    xmax = ex_m.data();
+  }
   const sf_problem &p0 = sl[0];
   pl.ctx = ctx;
   pl.n = n;
+  pl.nout = n_in;
   const bool flat = p0.geometry == SF_GEOM_FLAT;
   pl.G = flat ? kFlat : kSphere;
   pl.D = p0.D;
   pl.geom_code = p0.geometry;
   pl.J = p0.n_val > 0;
   pl.mixed = p0.precision != SF_PREC_FP64;
-  pl.ncell = out_cells(p0);
-  pl.sp = p0.op == SF_OP_SINGLE_PASS;
+  pl.ncell = out_cells(p0);  // per input slice
+  pl.sp = p0.op == SF_OP_SINGLE_PASS && !planes;
   pl.S64 = rec_d64(pl.G, pl.D);
   pl.F32 = rec_f32(pl.G, pl.D);
   const int G = pl.G, D = pl.D, ncell = pl.ncell, S64 = pl.S64, F32 = pl.F32;
@@ -821,7 +853,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
     Slice d = T.desc;
     d.N = sl[s].N;
     d.pt_off = prep[s].pt_off;
-    d.out_off = (int64_t)s * ncell;
+    d.out_off = planes ? (int64_t)(s / kSpRows) * ncell + s % kSpRows : (int64_t)s * ncell;
+    d.plane = planes ? s % kSpRows : 0;
     d.tab_off = (int32_t)cells.size();
     cells.insert(cells.end(), T.tab64.begin(), T.tab64.end());
     d.tab32_off = (int32_t)cells.size();
@@ -852,6 +885,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   Pk.op = p0.op;
   Pk.nl = p0.nl;
   Pk.nt = p0.nt;
+  Pk.n_val = p0.n_val;
   Pk.nmom = pl.sp ? 4 : 1;
   const int nmom = Pk.nmom;
   int t_block = ctx->tune_block, t_jchunk = ctx->tune_jchunk, t_bpc = ctx->tune_bpc;
@@ -985,8 +1019,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n, const sf_problem *sl, const double 
   HIPCHK(pl.dedges.ensure(dedges.size()));
   HIPCHK(pl.vedges.ensure(vedges.size()));
   HIPCHK(pl.vedges32.ensure(vedges32.size()));
-  HIPCHK(pl.acc_s.ensure((size_t)n * ncell));
-  HIPCHK(pl.acc_c.ensure((size_t)n * ncell));
+  HIPCHK(pl.acc_s.ensure((size_t)pl.nout * ncell));
+  HIPCHK(pl.acc_c.ensure((size_t)pl.nout * ncell));
   HIPCHK(pl.counter.ensure(4));
   HIPCHK(pl.pcount.ensure(4));
   HIPCHK(pl.flag.ensure(4));
@@ -1052,7 +1086,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
                 hipEvent_t *ev) {
   sf_ctx *ctx = pl.ctx;
   if (pl.total_pairs == 0) return SF_OK;
-  const int64_t ncells = (int64_t)pl.n * pl.ncell;
+  const int64_t ncells = (int64_t)pl.nout * pl.ncell;
   HIPCHK(launch_prep(pl.G, pl.D, use_mixed, pl.geom_code, pl.prep.p, pl.n, pl.max_n, pl.rec64.p,
                      pl.rec32.p, st));
   HIPCHK(launch_zero(pl.acc_s.p, sizeof(double) * ncells, st));
@@ -1077,7 +1111,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
   HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
                               use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
-  if (pl.sp) HIPCHK(launch_derive_sp(out_s, out_c, (int64_t)pl.n * pl.P.n_dist, st));
+  if (pl.sp) HIPCHK(launch_derive_sp(out_s, out_c, (int64_t)pl.nout * pl.P.n_dist, st));
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   return SF_OK;
 }
@@ -1176,7 +1210,7 @@ int host_entry(sf_ctx *ctx, int n, const sf_problem *sl, double *sums, uint64_t 
     sums[k] += hs[k];
     counts[k] += hc[k];
   }
-  if (p0.op == SF_OP_SINGLE_PASS) derive_sp_host(sums, counts, (int64_t)n * p0.n_dist);
+  if (p0.op == SF_OP_SINGLE_PASS && p0.n_val == 0) derive_sp_host(sums, counts, (int64_t)n * p0.n_dist);
   return SF_OK;
 }
 
@@ -1411,7 +1445,8 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
   Pk.op = p0.op;
   Pk.nl = p0.nl;
   Pk.nt = p0.nt;
-  Pk.nmom = p0.op == SF_OP_SINGLE_PASS ? 4 : 1;
+  Pk.n_val = p0.n_val;
+  Pk.nmom = p0.op == SF_OP_SINGLE_PASS && p0.n_val == 0 ? 4 : 1;
   const int G = p0.geometry == SF_GEOM_FLAT ? kFlat : kSphere;
   ctx->tuned[std::make_tuple(G, p0.D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val)] = best_c;
   if (out) std::tie(out[0], out[1], out[2], out[3]) = best_c;

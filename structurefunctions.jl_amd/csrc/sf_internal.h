@@ -65,7 +65,7 @@ struct Slice {
   // fixed-point joint accumulation: a value v of value bin g (0-based) is stored as the integer
   // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
   float vwf, vsf;
-  int32_t pad0;
+  int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
 };
 
 struct Item {
@@ -134,6 +134,10 @@ struct SmemLayout {
 // distance bin (lanes of one wave usually share a distance bin but not a value bin, so their
 // atomics then fall in different banks); 1D histograms keep one copy per lane group.
 inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
+// single pass, 2D form (op SF_OP_SINGLE_PASS with value bins): every input slice runs as six
+// plane slices, one per invariant, each a joint histogram written to every sixth output cell
+inline __host__ __device__ bool sp2d(int op, int n_val) { return op == 6 && n_val > 0; }
+
 inline constexpr int kDummySlots = 64;
 // largest re-bin queue staged in LDS per workgroup (entries); the host shrinks it where the LDS
 // it takes would cost a workgroup per CU, and entries past it go straight to the global queue

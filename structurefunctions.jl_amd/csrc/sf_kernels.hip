@@ -233,7 +233,25 @@ DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
 // operator at compile time (1: {1,0}, 2: {2,0}) with the same operation sequence as the generic
 // functor; 0 dispatches on P.op at run time; 3 is the single pass, which writes du_L to val[0] and
 // |du|^2 to val[1] (flat: _pf_sp_simd_pairs!, serial_single_pass.jl:245-253, du_L =
-// dot(du, dx) * inv(sqrt(r2)); sphere: _sp1d_pairs!, :333-338, du_L = dot(du, e1)).
+// dot(du, dx) * inv(sqrt(r2)); sphere: _sp1d_pairs!, :333-338, du_L = dot(du, e1)); 4 is a plane
+// of the single pass's 2D form: the same du_L and |du|^2, then invariant s.plane as value.
+// Invariant t of the single pass's 2D form from du_L and |du|^2, in the reference's operation
+// order (_sp2d_simd_pairs!, serial_single_pass.jl:906-911): |du|^2, du_L^2, |du|^2 - du_L^2,
+// du_L |du|^2, du_L du_L^2, du_L (|du|^2 - du_L^2).  t is uniform over a slice.
+template <typename T>
+DEV T sp_invariant(int t, T duL, T dn2) {
+  const T duL2 = duL * duL;
+  const T duT2 = dn2 - duL2;
+  switch (t) {
+    case 0: return dn2;
+    case 1: return duL2;
+    case 2: return duT2;
+    case 3: return duL * dn2;
+    case 4: return duL * duL2;
+    default: return duL * duT2;
+  }
+}
+
 template <int G, int D, bool J, int OPK = 0>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
                    const Cell *tab, const double *dE, const double *vE, double *val) {
@@ -250,11 +268,17 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     const double r2 = sdot<D>(dx, dx);
     db = dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
-    if constexpr (OPK == 3) {
+    if constexpr (OPK == 3 || OPK == 4) {
       const double inv_r = 1.0 / __builtin_sqrt(r2);
-      val[0] = sdot<D>(du, dx) * inv_r;
-      val[1] = sdot<D>(du, du);
-      return db - 1;
+      const double duL = sdot<D>(du, dx) * inv_r;
+      const double dn2 = sdot<D>(du, du);
+      if constexpr (OPK == 3) {
+        val[0] = duL;
+        val[1] = dn2;
+        return db - 1;
+      } else {
+        v = sp_invariant(s.plane, duL, dn2);
+      }
     } else if constexpr (OPK == 2) {
       const double p = sdot<D>(du, dx);
       v = p * p / r2;
@@ -278,6 +302,8 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       val[0] = sdot<D>(du, rh);
       val[1] = sdot<D>(du, du);
       return db - 1;
+    } else if constexpr (OPK == 4) {
+      v = sp_invariant(s.plane, sdot<D>(du, rh), sdot<D>(du, du));
     } else if constexpr (OPK == 1) v = 1.0 * sdot<D>(du, rh);
     else if constexpr (OPK == 2) {
       const double L = sdot<D>(du, rh);
@@ -467,6 +493,8 @@ DEV void hist_add_sp(const Smem &m, int h, T duL, T dn2) {
 }
 
 DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
+  // output cell c of the slice is out_off + c, or every sixth cell for a 2D single-pass plane
+  const int64_t ostride = sp2d(P.op, P.n_val) ? kSpRows : 1;
   if (m.fx) {
     // fixed-point joint cells: n = word >> 44, sum v = n a_g + (sum q - n kFxBias) / vsf
     for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
@@ -479,8 +507,9 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
         const unsigned long long sq = wd & ((1ull << kFxShift) - 1ull);
         const double a = (double)__builtin_fmaf((float)vb0, s.vwf, s.v0f);
         const double v = (double)n * a + ((double)sq - (double)n * (double)kFxBias) / (double)s.vsf;
-        atomicAdd(&P.counts[s.out_off + c], n);
-        atomicAdd(&P.sums[s.out_off + c], v);
+        const int64_t o = s.out_off + ostride * c;
+        atomicAdd(&P.counts[o], n);
+        atomicAdd(&P.sums[o], v);
       }
     }
     return;
@@ -528,8 +557,8 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
       }
     }
     if (n) {
-      atomicAdd(&P.counts[s.out_off + c], n);
-      atomicAdd(&P.sums[s.out_off + c], v);
+      atomicAdd(&P.counts[s.out_off + ostride * c], n);
+      atomicAdd(&P.sums[s.out_off + ostride * c], v);
     }
   }
 }
@@ -590,13 +619,11 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
   constexpr int S = Rec<G, D>::d64;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
-  __shared__ unsigned long long s_qbase;
   const int B = blockDim.x;
   int cur = -1;
   Slice s{};
   unsigned long long since_flush = 0;
   zero_hist(m);
-  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -759,10 +786,12 @@ template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
 DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri, int i, int j,
                 const float *rj, float *rn, const float4 *next, bool has_next, PendingAdd &pa) {
   constexpr int F = Rec<G, D>::f32;
-  // OPK 3 (single pass) needs du_L and |du|^2; on the sphere |du|^2 = du_L^2 + |du_T|^2
+  // OPK 3 (single pass) and 4 (a plane of its 2D form) need du_L and |du|^2; on the sphere
+  // |du|^2 = du_L^2 + |du_T|^2
   constexpr bool SP = (OPK == 3);
-  constexpr bool needT = (OPK == 0) || (SP && G == kSphere);
-  constexpr bool needN2 = (OPK == 0) || SP;
+  constexpr bool SP2 = (OPK == 4);
+  constexpr bool needT = (OPK == 0) || ((SP || SP2) && G == kSphere);
+  constexpr bool needN2 = (OPK == 0) || SP || SP2;
   const int nb = P.n_dist;
   const bool upper = !MASK || j > i;
   bool amb, amb_d;
@@ -877,11 +906,22 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const float L2 = L * L;
     amb = amb || !(n2 < __builtin_inff()) || !(__builtin_fabsf(L * n2) < __builtin_inff()) ||
           !(__builtin_fabsf(L * L2) < __builtin_inff());
+  } else if constexpr (SP2) {
+    v = sp_invariant(s.plane, L, n2);
   } else {
     v = op32<OPK>(P, D, L, T, tn2, n2);
   }
   if constexpr (J) {
-    const float dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+    // 2D single-pass planes: |dv| <= 32 * order * 2^-24 * Umax^order * amp, order 2 for |du|^2,
+    // du_L^2, du_T^2 and 3 for the products with du_L (the difference du_T^2 = |du|^2 - du_L^2 and
+    // the products add the errors of both factors, hence twice the coefficient of one operator)
+    float dv;
+    if constexpr (SP2) {
+      const float u2 = umax * umax;
+      dv = (s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * umax) * amp;
+    } else {
+      dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+    }
     // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
     amb = amb || !(umax >= 0x1p-30f);
     float gf;
@@ -1091,6 +1131,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
 // ---------------------------------------------------------------------------------------------
 // recheck: binary64 binning of queued pairs (global-memory tables, global atomics)
 // ---------------------------------------------------------------------------------------------
+// SP: the single pass; with J, one plane of its 2D form (output cells every sixth)
 template <int G, int D, bool J, bool SP>
 __global__ void __launch_bounds__(256) recheck(Params P) {
   // Each workgroup takes a contiguous range of the queue.  Entries of the range's first slice are
@@ -1098,7 +1139,9 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
   // roughly slice order, so these are few) go straight to the global accumulators.
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
-  constexpr int NM = SP ? 4 : 1;
+  constexpr bool SP1 = SP && !J;          // 1D single pass: four moments per distance bin
+  constexpr int NM = SP1 ? 4 : 1;
+  constexpr int64_t OSTR = SP ? kSpRows : 1;
   const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
   double *hs = (double *)smem_raw;           // NM sums per cell, moment-major
   unsigned *hc = (unsigned *)(hs + NM * nh);
@@ -1122,11 +1165,11 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
         Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
       }
       double v[2];
-      const int c = exact_cell<G, D, J, SP ? 3 : 0>(P, s, A, Bv, P.cells + s.tab_off,
-                                                    P.dedges + s.dedge_off, P.vedges + s.vedge_off, v);
+      const int c = exact_cell<G, D, J, SP ? (J ? 4 : 3) : 0>(P, s, A, Bv, P.cells + s.tab_off,
+                                                              P.dedges + s.dedge_off, P.vedges + s.vedge_off, v);
       if (c >= 0) {
         double mv[NM];
-        if constexpr (SP) {
+        if constexpr (SP1) {
           const double duL2 = v[0] * v[0];
           mv[0] = v[1];
           mv[1] = duL2;
@@ -1140,10 +1183,10 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
           for (int t = 0; t < NM; ++t) atomicAdd(&hs[t * nh + c], mv[t]);
           atomicAdd(&hc[c], 1u);
         } else {
-          const int64_t o = s.out_off + (SP ? (int64_t)kSpRows * c : (int64_t)c);
+          const int64_t o = s.out_off + OSTR * c;
           atomicAdd(&P.counts[o], 1ull);
 #pragma unroll
-          for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP ? sp_row(t) : 0)], mv[t]);
+          for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP1 ? sp_row(t) : 0)], mv[t]);
         }
       }
     }
@@ -1151,10 +1194,10 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
     const int64_t off = P.slices[home].out_off;
     for (int c = threadIdx.x; c < nh; c += blockDim.x) {
       if (hc[c]) {
-        const int64_t o = off + (SP ? (int64_t)kSpRows * c : (int64_t)c);
+        const int64_t o = off + OSTR * c;
         atomicAdd(&P.counts[o], (unsigned long long)hc[c]);
 #pragma unroll
-        for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP ? sp_row(t) : 0)], hs[t * nh + c]);
+        for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP1 ? sp_row(t) : 0)], hs[t * nh + c]);
       }
     }
   }
@@ -1261,6 +1304,8 @@ template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
+  } else {
+    if (opk == 4) return (const void *)&hist32<G, D, true, 4, SL, FX>;
   }
   if (opk == 1) return (const void *)&hist32<G, D, J, 1, SL, FX>;
   if (opk == 2) return (const void *)&hist32<G, D, J, 2, SL, FX>;
@@ -1281,6 +1326,8 @@ static const void *hist_fn(bool mixed, int opk, bool sload, bool fx) {
   if (mixed) return sload ? hist32_fx<G, D, J, true>(fx, opk) : hist32_fx<G, D, J, false>(fx, opk);
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist64<G, D, false, 3>;
+  } else {
+    if (opk == 4) return (const void *)&hist64<G, D, true, 4>;
   }
   if (opk == 1) return (const void *)&hist64<G, D, J, 1>;
   if (opk == 2) return (const void *)&hist64<G, D, J, 2>;
@@ -1297,6 +1344,7 @@ static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl,
 }
 
 static int opk_of(const Params &P) {
+  if (sp2d(P.op, P.n_val)) return 4;
   if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
   if (P.op == 0 && P.nl == 2 && P.nt == 0) return 2;
@@ -1326,19 +1374,19 @@ hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Param
 
 template <int G, int D>
 static const void *recheck_fn(bool J, bool sp) {
-  if (J) return (const void *)&recheck<G, D, true, false>;
+  if (J) return sp ? (const void *)&recheck<G, D, true, true> : (const void *)&recheck<G, D, true, false>;
   return sp ? (const void *)&recheck<G, D, false, true> : (const void *)&recheck<G, D, false, false>;
 }
 
 hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st) {
-  const bool sp = P.nmom > 1;
+  const bool sp = P.nmom > 1 || sp2d(P.op, P.n_val);
   const void *f;
   if (G == kFlat) f = D == 2 ? recheck_fn<kFlat, 2>(J, sp) : recheck_fn<kFlat, 3>(J, sp);
   else f = D == 2 ? recheck_fn<kSphere, 2>(J, sp) : recheck_fn<kSphere, 3>(J, sp);
   Params Pc = P;
   void *args[] = {&Pc};
   const int nh = P.n_dist * (P.n_val > 0 ? P.n_val : 1);
-  const size_t smem = (size_t)nh * (sp ? 36 : 12) + 16;
+  const size_t smem = (size_t)nh * (P.nmom > 1 ? 36 : 12) + 16;
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   return hipLaunchKernel(f, dim3(grid), dim3(256), args, smem, st);
 }

@@ -8,7 +8,10 @@
 # (src/Calculations/dispatch.jl:377-477, 510-594), and the two single-pass hooks
 #   _dispatch_single_pass(backend, shape::PointField, x, u, distance_bins; ...)
 #   _dispatch_single_pass!(backend, sums, counts, x, u, distance_bins; ...)
-# (src/Calculations/serial_single_pass.jl:400-535).  The public entry points validate shapes and
+# (src/Calculations/serial_single_pass.jl:400-535), and the 2D single-pass hooks
+#   _dispatch_single_pass_2d(backend, shape::PointField, x, u, distance_bins, value_bins; ...)
+#   _dispatch_single_pass_2d!(backend, sums_3d, counts_3d, x, u, distance_bins, value_bins; ...)
+# (serial_single_pass.jl:935-1105).  The public entry points validate shapes and
 # count width and apply `_finalize` around these calls (dispatch.jl:64-136, 346-373), exactly as
 # for the Serial/Threaded/GPU backends.
 #
@@ -224,6 +227,68 @@ function SFC._dispatch_single_pass!(
         end
     end
     return sums, counts
+end
+
+# ---- single pass, 2D form: six joint (distance, value) histograms (serial_single_pass.jl:694-1181)
+# value_bins is one vector shared by the invariants or a 6-tuple (SinglePass2DValueBins); only the
+# first n_val + 1 edges of each matter (a value above them falls in a bin > n_val, which the
+# reference drops, _sp2d_scatter!, :852-864), so the library gets those, back to back.
+_sp2d_vector(value_bins, t) = value_bins isa Tuple ? value_bins[t] : value_bins
+
+function _run_sp2d!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, x, u,
+                    dbins, value_bins, n_val::Int, metric, CT)
+    ctx = _context(b.device)
+    xc = convert(Matrix{Float64}, x)
+    uc = convert(Matrix{Float64}, u)
+    de = _edges(dbins)
+    ve = reduce(vcat, (_edges(_sp2d_vector(value_bins, t))[1:(n_val + 1)] for t in 1:6))
+    geom, radius = _geometry(metric)
+    GC.@preserve xc uc de ve sums counts begin
+        p = SfProblem(Int32(size(uc, 1)), Int32(size(xc, 1)), Int64(size(xc, 2)),
+                      pointer(xc), pointer(uc), geom, radius, Int32(6), Int32(0), Int32(0),
+                      Int32(length(de) - 1), pointer(de), Int32(n_val), pointer(ve),
+                      _prec(b), _count_bits(CT))
+        rc = ccall((:sf_hist, libsfmi355x), Cint,
+                   (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
+                   ctx, p, sums, counts)
+        _check(ctx, rc)
+    end
+    return nothing
+end
+
+function SFC._dispatch_single_pass_2d(
+    b::MI355XBackend, ::SFC.PointField, x::AbstractMatrix, u::AbstractMatrix,
+    distance_bins::AbstractVector, value_bins::SFC.SinglePass2DValueBins;
+    count_eltype::Type{CT} = UInt32, distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {CT}
+    nb = length(distance_bins) - 1
+    n_val = length(_sp2d_vector(value_bins, 1)) - 1
+    SFC._validate_value_bins!(value_bins, n_val)
+    sums = zeros(Float64, 6 * nb * n_val)
+    counts = zeros(UInt64, 6 * nb * n_val)
+    _run_sp2d!(b, sums, counts, x, u, distance_bins, value_bins, n_val, distance_metric, CT)
+    return (sums = reshape(sums, 6, nb, n_val), counts = CT.(reshape(counts, 6, nb, n_val)))
+end
+
+function SFC._dispatch_single_pass_2d!(
+    b::MI355XBackend, sums_3d::AbstractArray, counts_3d::AbstractArray, x::AbstractMatrix,
+    u::AbstractMatrix, distance_bins::AbstractVector, value_bins::SFC.SinglePass2DValueBins;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+)
+    # the shape checks of _accumulate_single_pass_2d! (serial_single_pass.jl:740-747)
+    nb = length(distance_bins) - 1
+    size(sums_3d, 1) == 6 && size(sums_3d, 2) == nb ||
+        throw(DimensionMismatch("sums must have shape (6, n_bins, n_val); got $(size(sums_3d))"))
+    size(counts_3d) == size(sums_3d) ||
+        throw(DimensionMismatch("counts and sums must have the same shape"))
+    n_val = size(sums_3d, 3)
+    SFC._validate_value_bins!(value_bins, n_val)
+    s = zeros(Float64, length(sums_3d))
+    c = zeros(UInt64, length(counts_3d))
+    _run_sp2d!(b, s, c, x, u, distance_bins, value_bins, n_val, distance_metric, eltype(counts_3d))
+    sums_3d .+= reshape(s, size(sums_3d))
+    counts_3d .+= reshape(c, size(counts_3d))
+    return sums_3d, counts_3d
 end
 
 end # module

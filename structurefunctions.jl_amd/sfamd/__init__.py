@@ -15,6 +15,8 @@ from .objects import (StructureFunction, StructureFunction2DSumsAndCounts, Struc
 from .operators import *  # noqa: F401,F403
 from .operators import get_structure_function_type, order
 from .single_pass import (SINGLE_PASS_N, SINGLE_PASS_OPERATORS, calculate_structure_functions_single_pass,
+                          calculate_structure_functions_single_pass_2d,
+                          calculate_structure_functions_single_pass_2d_inplace,
                           calculate_structure_functions_single_pass_inplace)
 
 calculate_structure_function_bang = calculate_structure_function_inplace

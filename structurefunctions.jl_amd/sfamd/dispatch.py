@@ -125,6 +125,9 @@ def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep):
     p.n_dist, p.dist_edges = len(dedges) - 1, dedges.ctypes.data
     if vedges is None:
         p.n_val, p.val_edges = 0, None
+    elif kind == _native.OP_SINGLE_PASS:
+        # the single pass's 2D form: six edge vectors back to back
+        p.n_val, p.val_edges = len(vedges) // 6 - 1, vedges.ctypes.data
     else:
         p.n_val, p.val_edges = len(vedges) - 1, vedges.ctypes.data
     p.precision = precision
@@ -150,8 +153,12 @@ def _run(sf, slices, dedges_list, vedges_list, metric, backend):
     problems = [_problem(sf, xs, us, de, ve, metric, backend.precision_code, keep)
                 for (xs, us), de, ve in zip(slices, dedges_list, vedges_list)]
     nd = len(dedges_list[0]) - 1
-    nv = 0 if vedges_list[0] is None else len(vedges_list[0]) - 1
-    ncell = 6 * nd if sf.code[0] == _native.OP_SINGLE_PASS else nd * max(nv, 1)
+    sp = sf.code[0] == _native.OP_SINGLE_PASS
+    if vedges_list[0] is None:
+        nv = 0
+    else:
+        nv = (len(vedges_list[0]) // 6 if sp else len(vedges_list[0])) - 1
+    ncell = (6 if sp else 1) * nd * max(nv, 1)
     sums = np.zeros(len(slices) * ncell, dtype=np.float64)
     counts = np.zeros(len(slices) * ncell, dtype=np.uint64)
     backend.context().hist(problems, sums, counts)

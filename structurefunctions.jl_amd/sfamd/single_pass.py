@@ -7,6 +7,14 @@
         calculate_structure_functions_single_pass! (serial_single_pass.jl:404-421): accumulates
         into (6, n_bins[, auxiliary...]) buffers
 
+    calculate_structure_functions_single_pass_2d(x, u, distance_bins, value_bins; ...)
+        src/Calculations/serial_single_pass.jl:1161-1181; dict keyed by invariant of
+        StructureFunction2DSumsAndCounts (each invariant binned by distance and its own value)
+    calculate_structure_functions_single_pass_2d_inplace(sums_3d, counts_3d, x, u, distance_bins,
+                                                         value_bins; ...)
+        calculate_structure_functions_single_pass_2d! (serial_single_pass.jl:712-728): accumulates
+        into (6, n_bins, n_val[, auxiliary...]) buffers
+
 One pass bins every pair by distance and accumulates |du|^2, du_L^2, du_L |du|^2 and du_L^3
 (rows 1, 2, 4, 5); T2 = S2 - L2 and L1T2 = S3 - L3 are differenced per bin and the shared count is
 replicated into every row (_sp1d_derive_rows!, serial_single_pass.jl:208-219).  The pair loop is
@@ -23,7 +31,7 @@ from .dispatch import (MI355XBackend, _assert_counts_representable, _finalize, _
                        _validate_array_shape)
 from .errors import DimensionMismatch
 from .metrics import Euclidean
-from .objects import StructureFunction, StructureFunctionSumsAndCounts
+from .objects import StructureFunction, StructureFunction2DSumsAndCounts, StructureFunctionSumsAndCounts
 from .operators import L1T2SF, L2SF, L3SF, S2SF, S3SF, T2SF
 
 SINGLE_PASS_N = 6
@@ -92,3 +100,92 @@ def calculate_structure_functions_single_pass_inplace(sums, counts, x, u, distan
     for r in range(1, SINGLE_PASS_N):
         counts[r] = counts[0]
     return sums, counts
+
+
+# ---- 2D (joint) form ------------------------------------------------------------------------
+
+def _value_bin_list(value_bins):
+    """The six per-invariant edge vectors: one vector shared by all invariants, or a 6-tuple
+    (SinglePass2DValueBins, batch_api.jl:101-104)."""
+    if isinstance(value_bins, (tuple, list)) and len(value_bins) == SINGLE_PASS_N and \
+            all(np.ndim(v) == 1 for v in value_bins):
+        return list(value_bins)
+    return [value_bins] * SINGLE_PASS_N
+
+
+def _validate_value_bins(vlist, n_val):
+    """_validate_value_bins! (batch_api.jl:132-150): every vector needs n_val + 1 edges or more."""
+    for t, v in enumerate(vlist):
+        if len(v) < n_val + 1:
+            raise DimensionMismatch(
+                f"value_bins[{t + 1}] needs at least {n_val + 1} edges for n_val={n_val} (got {len(v)})")
+
+
+def _single_pass_2d_raw(x, u, distance_bins, vlist, n_val, backend, distance_metric):
+    """(sums, counts) shaped (6, n_bins, n_val, auxiliary...) for fresh buffers.  Only the first
+    n_val + 1 edges of each vector matter: a value above edge n_val + 1 falls in a bin > n_val,
+    which the reference drops (_sp2d_scatter!, serial_single_pass.jl:852-864)."""
+    backend = backend or MI355XBackend()
+    kind, _, _ = _validate_array_shape(x, u, distance_metric)
+    de = as_edges(distance_bins)
+    ve6 = np.ascontiguousarray(np.concatenate(
+        [np.asarray(as_edges(v), dtype=np.float64)[:n_val + 1] for v in vlist]))
+    slices, aux = _slices(x, u, kind)
+    sums, counts, nd, nv = _run(_SinglePassOp(), slices, [de] * len(slices), [ve6] * len(slices),
+                                distance_metric, backend)
+    B = len(slices)
+    ncell = SINGLE_PASS_N * nd * nv
+    s = sums.reshape(B, ncell).T.reshape((SINGLE_PASS_N, nd, nv) + aux, order="F")
+    c = counts.reshape(B, ncell).T.reshape((SINGLE_PASS_N, nd, nv) + aux, order="F")
+    return s, c
+
+
+def calculate_structure_functions_single_pass_2d(x, u, distance_bins, value_bins, *,
+                                                 backend: MI355XBackend | None = None,
+                                                 output_type=StructureFunction2DSumsAndCounts,
+                                                 count_eltype=np.uint32, distance_metric=Euclidean()):
+    """Six invariant joint (distance, value) histograms in one pass, as {name: result}
+    (serial_single_pass.jl:1161-1181, collection :1139-1150).  Unlike the 1D form the counts
+    differ per invariant: each invariant's value lands in its own value bin."""
+    x = np.asarray(x, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    _validate_array_shape(x, u, distance_metric)
+    _assert_counts_representable(count_eltype, x.shape[1])
+    vlist = _value_bin_list(value_bins)
+    n_val = len(vlist[0]) - 1
+    _validate_value_bins(vlist, n_val)
+    s, c = _single_pass_2d_raw(x, u, distance_bins, vlist, n_val, backend, distance_metric)
+    c = c.astype(count_eltype)
+    out = {}
+    for t, (name, op) in enumerate(SINGLE_PASS_OPERATORS.items()):
+        raw = StructureFunction2DSumsAndCounts(op, distance_bins, vlist[t], s[t], c[t])
+        out[name] = _finalize(raw, output_type)
+    return out
+
+
+def calculate_structure_functions_single_pass_2d_inplace(sums_3d, counts_3d, x, u, distance_bins,
+                                                         value_bins, *,
+                                                         backend: MI355XBackend | None = None,
+                                                         distance_metric=Euclidean()):
+    """calculate_structure_functions_single_pass_2d! (serial_single_pass.jl:712-728, checks of
+    _accumulate_single_pass_2d! :740-747): accumulates into (6, n_bins, n_val[, aux...]) buffers,
+    n_val = size(sums_3d, 3)."""
+    x = np.asarray(x, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    kind, _, _ = _validate_array_shape(x, u, distance_metric)
+    nd = len(as_edges(distance_bins)) - 1
+    shp = tuple(np.shape(sums_3d))
+    if len(shp) < 3 or shp[0] != SINGLE_PASS_N or shp[1] != nd:
+        raise DimensionMismatch(f"sums must have shape ({SINGLE_PASS_N}, n_bins, n_val); got {shp}")
+    if tuple(np.shape(counts_3d)) != shp:
+        raise DimensionMismatch("counts and sums must have the same shape")
+    n_val = shp[2]
+    aux = () if kind == "point" else u.shape[2:]
+    if shp[3:] != tuple(aux):
+        raise DimensionMismatch(f"sums must have shape {(SINGLE_PASS_N, nd, n_val) + tuple(aux)}; got {shp}")
+    vlist = _value_bin_list(value_bins)
+    _validate_value_bins(vlist, n_val)
+    s, c = _single_pass_2d_raw(x, u, distance_bins, vlist, n_val, backend, distance_metric)
+    sums_3d += s
+    counts_3d += c.astype(counts_3d.dtype)
+    return sums_3d, counts_3d

@@ -40,6 +40,11 @@ def lib():
                                            ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                            ctypes.c_void_p]
         L.sfo_hist_single_pass.restype = ctypes.c_int
+        L.sfo_hist_single_pass_2d.argtypes = [ctypes.POINTER(Prob), ctypes.c_int, ctypes.c_int64,
+                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]
+        L.sfo_hist_single_pass_2d.restype = ctypes.c_int
         L.sfo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
         L.sfo_atan2.restype = ctypes.c_double
         L.sfo_sincosd.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -108,6 +113,40 @@ def single_pass(x, u, edges, geometry="flat", radius=1.0, nthreads=1, rows=None)
     if rc != 0:
         raise ValueError(f"oracle rejected the problem (code {rc})")
     return s.reshape((6, nd), order="F"), c.reshape((6, nd), order="F")
+
+
+def single_pass_2d(x, u, edges, vedges, geometry="flat", radius=1.0, nthreads=1, rows=None,
+                   with_mag=False):
+    """Single pass, 2D form: (sums, counts), each (6, n_dist, n_val); with_mag adds sum |value| and
+    sum ||du||^order per cell.  vedges is one edge vector shared by the six invariants or a
+    sequence of six (each at least n_val + 1 long)."""
+    if len(vedges) == 6 and np.ndim(vedges[0]) == 1:
+        # n_val from the first vector; the others need at least n_val + 1 edges and only their
+        # first n_val + 1 matter (_validate_value_bins!, batch_api.jl:132-150)
+        nv = len(vedges[0]) - 1
+        ve6 = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.float64)[:nv + 1] for v in vedges]))
+    else:
+        v = np.asarray(vedges, dtype=np.float64)
+        nv = len(v) - 1
+        ve6 = np.ascontiguousarray(np.tile(v, 6))
+    assert ve6.size == 6 * (nv + 1)
+    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None)
+    p.n_val = nv
+    nd = p.n_dist
+    s = np.zeros(6 * nd * nv)
+    c = np.zeros(6 * nd * nv, dtype=np.uint64)
+    a = np.zeros(6 * nd * nv)
+    g = np.zeros(6 * nd * nv)
+    i0, i1, st = (0, p.N, 1) if rows is None else rows
+    rc = lib().sfo_hist_single_pass_2d(ctypes.byref(p), nthreads, i0, i1, st, ve6.ctypes.data,
+                                       s.ctypes.data, c.ctypes.data, a.ctypes.data, g.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the problem (code {rc})")
+    shp = (6, nd, nv)
+    out = (s.reshape(shp, order="F"), c.reshape(shp, order="F"))
+    if with_mag:
+        out += (a.reshape(shp, order="F"), g.reshape(shp, order="F"))
+    return out
 
 
 def pair(x, u, edges, i, j, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):

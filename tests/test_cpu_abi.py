@@ -66,14 +66,26 @@ def test_check_rejects(kw, code):
 
 
 def test_single_pass_checks():
-    """SF_OP_SINGLE_PASS is accepted for distance bins and refused with value bins."""
+    """SF_OP_SINGLE_PASS: distance bins only (1D form), or with value bins the 2D form, whose
+    val_edges holds six edge vectors back to back, each validated (sorted, no NaN)."""
     p, keep = prob(op=_native.OP_SINGLE_PASS, nl=0)
     assert _native.check_problems([p])[0] == 0
-    ve = np.linspace(-1, 1, 5)
-    keep.append(ve)
-    p.n_val, p.val_edges = 4, ve.ctypes.data
+    ve6 = np.concatenate([np.linspace(-1, 1, 5)] * 6)
+    keep.append(ve6)
+    p.n_val, p.val_edges = 4, ve6.ctypes.data
+    assert _native.check_problems([p])[0] == 0
+    bad = ve6.copy()
+    bad[5 * 5 + 2] = np.nan            # the last invariant's vector
+    keep.append(bad)
+    p.val_edges = bad.ctypes.data
     rc, msg = _native.check_problems([p])
-    assert rc == _native.SF_E_UNSUPPORTED and "single-pass" in msg
+    assert rc == _native.SF_E_ARG and "NaN" in msg
+    bad2 = ve6.copy()
+    bad2[3 * 5 + 1], bad2[3 * 5 + 2] = 0.9, -0.9   # the fourth vector unsorted
+    keep.append(bad2)
+    p.val_edges = bad2.ctypes.data
+    rc, msg = _native.check_problems([p])
+    assert rc == _native.SF_E_ARG and "sorted" in msg
 
 
 def test_count_width_guard():

@@ -89,3 +89,59 @@ def test_single_pass_accumulates_then_derives():
     np.testing.assert_array_equal(s[5], s[3] - s[4])
     for r in range(6):
         np.testing.assert_array_equal(c[r], 2 * c1[0])
+
+
+SP_ROWS = [(1, 0, 0), (0, 2, 0), (0, 0, 2), (2, 0, 0), (0, 3, 0), (0, 1, 2)]
+
+
+def sp2d_value_edges(geometry):
+    """Per-invariant value edges: log-spaced for the non-negative invariants, linear for the
+    signed ones (the pairing the reference's docs suggest, batch_api.jl:108-112)."""
+    pos = np.concatenate([[0.0], np.geomspace(0.05, 12.0, 6)])
+    sgn = np.array([-9.0, -2.2, -0.4, 0.3, 1.9, 8.5, 40.0])
+    return [pos, pos, pos, sgn, sgn, sgn]
+
+
+@pytest.mark.parametrize("geometry,D", [("flat", 2), ("flat", 3), ("deg", 2), ("deg", 3)])
+def test_single_pass_2d_vs_mpmath(geometry, D):
+    """Single pass, 2D form (serial_single_pass.jl:1161-1181): invariant t's (distance, value)
+    histogram equals the 40-digit brute force of its operator binned by its own value edges."""
+    x, u, edges = case(geometry, D, 15, seed=500 + D)
+    ve = sp2d_value_edges(geometry)
+    s, c = _oracle.single_pass_2d(x, u, edges, ve, geometry=geometry)
+    for t, op in enumerate(SP_ROWS):
+        ms, mc, ma, margin = mp_reference.brute(x, u, edges, op, geometry, vedges=ve[t])
+        assert margin > 1e-9, (t, margin)
+        assert np.array_equal(c[t], mc), t
+        assert np.all(np.abs(s[t] - ms) <= 1e-12 * np.maximum(ma, 1e-300) + 1e-300), t
+
+
+def test_single_pass_2d_marginals_match_1d():
+    """With value edges wide enough to hold every value, summing the 2D form over value bins
+    gives the 1D single pass: counts exactly, sums to rounding (T2 and L1T2 are accumulated per
+    pair here and derived from the other rows in 1D)."""
+    x, u, edges = case("flat", 2, 40, seed=9)
+    wide = np.array([-1e300, -1.0, 0.0, 0.5, 2.0, 1e300])
+    s2, c2 = _oracle.single_pass_2d(x, u, edges, wide)
+    s1, c1 = _oracle.single_pass(x, u, edges)
+    for t in range(6):
+        np.testing.assert_array_equal(c2[t].sum(axis=1), c1[t])
+    scale = np.abs(s1).max()
+    np.testing.assert_allclose(s2.sum(axis=2), s1, rtol=0, atol=1e-12 * scale)
+    # one shared vector == six copies of it
+    s6, c6 = _oracle.single_pass_2d(x, u, edges, [wide] * 6)
+    np.testing.assert_array_equal(c6, c2)
+    np.testing.assert_array_equal(s6, s2)
+
+
+def test_single_pass_2d_threads_and_rows():
+    """Round-robin threads give the same counts and sums to rounding; row subsets add up."""
+    x, u, edges = case("deg", 2, 60, seed=11)
+    ve = sp2d_value_edges("deg")
+    s1, c1 = _oracle.single_pass_2d(x, u, edges, ve, geometry="deg")
+    s4, c4 = _oracle.single_pass_2d(x, u, edges, ve, geometry="deg", nthreads=4)
+    np.testing.assert_array_equal(c1, c4)
+    np.testing.assert_allclose(s1, s4, rtol=1e-13, atol=1e-300)
+    sa, ca = _oracle.single_pass_2d(x, u, edges, ve, geometry="deg", rows=(0, 60, 2))
+    sb, cb = _oracle.single_pass_2d(x, u, edges, ve, geometry="deg", rows=(1, 60, 2))
+    np.testing.assert_array_equal(ca + cb, c1)
