@@ -212,6 +212,8 @@ struct sf_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool ev_pending = false;
   int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
+  // largest re-bin queue a call of this context has needed (entries): later plans start there
+  int64_t queue_hint = 0;
   // autotuner cache: (geometry, D, joint, mixed, operator class, log2 pairs, n_dist, n_val) ->
   // (block, j_chunk, blocks_per_cu, variant)
   std::map<std::tuple<int, int, int, int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;
@@ -855,6 +857,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     d.pt_off = prep[s].pt_off;
     d.out_off = planes ? (int64_t)(s / kSpRows) * ncell + s % kSpRows : (int64_t)s * ncell;
     d.plane = planes ? s % kSpRows : 0;
+    d.xerr = flat ? round_up_f(ldexpl((long double)xmax[s], -28)) : 0.0f;
+    d.pad1 = 0;
     d.tab_off = (int32_t)cells.size();
     cells.insert(cells.end(), T.tab64.begin(), T.tab64.end());
     d.tab32_off = (int32_t)cells.size();
@@ -1037,6 +1041,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   if (mixed) {
     pl.cap = std::max<int64_t>(1LL << 22, pl.total_pairs / 2000);
     pl.cap = std::min<int64_t>(pl.cap, 1LL << 27);
+    pl.cap = std::max<int64_t>(pl.cap, std::min<int64_t>(ctx->queue_hint, kQueueMax));
     HIPCHK(pl.pending.ensure((size_t)pl.cap));
   }
 
@@ -1140,6 +1145,20 @@ int run_batch(sf_ctx *ctx, int n, const sf_problem *sl, const double *const *dx,
     HIPCHK(hipMemcpyAsync(&pc, pl.pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats[1] = (int64_t)pc;
+    if ((int64_t)pc > pl.cap && (int64_t)pc <= kQueueMax) {
+      // the merge was skipped: the queue now knows its size, so grow it and run the binary32
+      // kernel once more (and remember the size for later calls of this context)
+      ctx->queue_hint = std::max<int64_t>(ctx->queue_hint, std::min<int64_t>((int64_t)pc + pc / 4, kQueueMax));
+      pl.cap = ctx->queue_hint;
+      HIPCHK(pl.pending.ensure((size_t)pl.cap));
+      pl.P.pending = pl.pending.p;
+      pl.P.pending_cap = pl.cap;
+      rc = plan_launch(pl, true, out_s, out_c, st, ctx->ev);
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(&pc, pl.pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      ctx->stats[1] = (int64_t)pc;
+    }
     if ((int64_t)pc > pl.cap) {
       // the merge was skipped; redo everything with the binary64 kernel (which evaluates every
       // pair in the caller's order: rebuild the work items without culling first)

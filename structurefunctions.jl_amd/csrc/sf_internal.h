@@ -66,6 +66,8 @@ struct Slice {
   // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
   float vwf, vsf;
   int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
+  float xerr;         // flat: 2^-28 max|x| (binary32 value bound: reference's direction error)
+  int32_t pad1;
 };
 
 struct Item {
@@ -74,7 +76,7 @@ struct Item {
   int32_t j0, j1;   // column range [j0, j1), j0 >= i0
 };
 
-struct Pending {      // a pair the binary32 kernel could not bin with certainty
+struct Pending {      // a pair the binary32 kernel could not bin with certainty (binary32 record indices)
   int32_t slice, i, j, pad;
 };
 
@@ -142,6 +144,9 @@ inline constexpr int kDummySlots = 64;
 // largest re-bin queue staged in LDS per workgroup (entries); the host shrinks it where the LDS
 // it takes would cost a workgroup per CU, and entries past it go straight to the global queue
 inline constexpr int kQueueCap = 1024;
+// largest global re-bin queue (entries of 16 bytes: 4 GiB); a call needing more is redone with
+// the binary64 kernel
+inline constexpr int64_t kQueueMax = 1LL << 28;
 
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
                                                   int stage_bytes, int copies, int nmom, int fx,

@@ -844,6 +844,10 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   bool vb_ok = true;
   float v = 0.f;
   int vb0 = 0;
+  // value bound inputs (joint histograms): largest |component| of the velocity difference, the
+  // flat direction-error factor, and the floor of the difference scale (below)
+  float mx = 0.f, xamp = 1.0f, dfloor = 0.f;
+  constexpr bool TIGHT = J && (OPK != 1);
   if constexpr (G == kFlat) {
 #pragma unroll
     for (int k = 0; k < D; ++k) du[k] = (rj[2 * D + k] - ri[2 * D + k]) + (rj[3 * D + k] - ri[3 * D + k]);
@@ -852,6 +856,16 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
     const float rinv = frsq(key);
     L = p * rinv;
+    if constexpr (J) {
+      // the reference's binary64 dx carries an absolute error ~2^-53 max|x|, which turns r's
+      // direction by that much over |dx|: a relative error of its du_L of 2^-53 max|x| / |dx|
+      xamp = __builtin_fmaf(s.xerr, rinv, 1.0f);
+      if constexpr (TIGHT) {
+        mx = __builtin_fmaxf(__builtin_fabsf(du[0]), __builtin_fabsf(du[1]));
+        if constexpr (D == 3) mx = __builtin_fmaxf(mx, __builtin_fabsf(du[2]));
+        dfloor = 0x1p-20f * umax;
+      }
+    }
     if constexpr (needN2) {
       n2 = du[0] * du[0];
 #pragma unroll
@@ -888,11 +902,12 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     amb = amb || !(s2 > 1e-30f);
     const float inv_s = frsq(s2);
     L = __builtin_fmaf(d[2], dU[2], __builtin_fmaf(d[1], dU[1], d[0] * dU[0])) * inv_s;
+    float dw = 0.f;
     if constexpr (needT) {
       T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
       tn2 = T * T;
       if constexpr (D == 3) {
-        const float dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
+        dw = (rj[13] - ri[13]) + (rj[14] - ri[14]);
         tn2 = __builtin_fmaf(dw, dw, tn2);
       }
       n2 = __builtin_fmaf(L, L, tn2);
@@ -900,6 +915,12 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // binary32 error of du_L, du_T grows like (1 + c^2 / sin(sigma)): near antipodal pairs the
     // numerators cancel and sin(sigma) is small
     amp = __builtin_fmaf(key, inv_s, 1.0f);
+    if constexpr (TIGHT) {
+      mx = __builtin_fmaxf(__builtin_fmaxf(__builtin_fabsf(dU[0]), __builtin_fabsf(dU[1])), __builtin_fabsf(dU[2]));
+      if constexpr (needT && D == 3) mx = __builtin_fmaxf(mx, __builtin_fabsf(dw));
+      // the reference's own binary64 frame error for close points grows like 2^-53 / sin(sigma)
+      dfloor = umax * __builtin_fmaf(0x1p-26f, inv_s, 0x1p-20f);
+    }
   }
   if constexpr (SP) {
     // moments are formed at the accumulation; all four must be finite in binary32
@@ -912,15 +933,28 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     v = op32<OPK>(P, D, L, T, tn2, n2);
   }
   if constexpr (J) {
-    // 2D single-pass planes: |dv| <= 32 * order * 2^-24 * Umax^order * amp, order 2 for |du|^2,
-    // du_L^2, du_T^2 and 3 for the products with du_L (the difference du_T^2 = |du|^2 - du_L^2 and
-    // the products add the errors of both factors, hence twice the coefficient of one operator)
+    // |v - v_ref| <= coef * 2^-24 * m^order * amp, m bounding |du| and the binary32 error's
+    // scale.  m = ||u_i|| + ||u_j|| always holds.  Where the velocities differ little (close
+    // points of a smooth field) the error scales with the difference instead: with dU the
+    // difference of the hi/lo velocity records, |du_L| <= |dU| amp and |du| <= sqrt(3) max|dU_k|
+    // (flat) or 3 max(|dU_k|, |du_r|) (sphere), plus a floor of 2^-20 (||u_i|| + ||u_j||) that
+    // covers the 2^-48 relative error of the hi/lo split and the reference's own binary64
+    // rounding (checked by a binary32 emulation against binary64 over near, far, near-antipodal
+    // pairs and differences down to 1e-13 of the velocities: at most 0.2 of the coefficients
+    // below).  Not for the du_L kernel, whose linear value bins gain nothing from it.
+    float mm = umax;
+    const float ampv = (G == kFlat) ? xamp : amp;
+    if constexpr (TIGHT)
+      mm = __builtin_fminf(umax, __builtin_fmaf(G == kFlat ? 1.7320509f : 3.0f, mx, dfloor) * ampv);
+    // 2D single-pass planes: coefficient 32 * order, order 2 for |du|^2, du_L^2, du_T^2 and 3
+    // for the products with du_L (the difference du_T^2 = |du|^2 - du_L^2 and the products add
+    // the errors of both factors, hence twice the coefficient of one operator)
     float dv;
     if constexpr (SP2) {
-      const float u2 = umax * umax;
-      dv = (s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * umax) * amp;
+      const float u2 = mm * mm;
+      dv = (s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * mm) * ampv;
     } else {
-      dv = P.vcoef * pow_order<OPK>(P, umax) * amp;
+      dv = P.vcoef * pow_order<OPK>(P, mm) * ampv;
     }
     // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
     amb = amb || !(umax >= 0x1p-30f);
@@ -952,34 +986,64 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   return defer;
 }
 
-// Queue the pairs (i, jg + t) whose bit t is set in dm for the binary64 re-bin.  Kept out of the
-// pair loop: a branch there (with its global atomic and lane exchange) splits the loop body into
-// pieces the compiler cannot overlap.  Entries are staged in the workgroup's LDS queue and moved
-// to the global queue once per work item (queue_flush): a returning global atomic on the one
-// queue counter, shared by every wave of the device, costs thousands of cycles under that
-// contention.  Entries past the LDS capacity go to the global queue directly.  Pairs are queued
-// in the caller's order (records may be permuted or spatially sorted), so the binary64 re-bin
-// evaluates the pair as the reference does, with i < j.
-template <int G, int D>
-DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, const float *ri,
-                        const float4 *col, int col_j0, int jg, int slice_id) {
-  constexpr int R = Rec<G, D>::f32 / 4;
-  if (dm == 0u) return;
-  unsigned k = m.qcap > 0 ? atomicAdd(m.qn, (unsigned)__popc(dm)) : 0u;
-  const int oi = __float_as_int(ri[Rec<G, D>::id]);
+// Exclusive prefix sum of n (< 64) over the active lanes of the wave, from one ballot per bit;
+// *total receives the sum.  Works with any set of active lanes.
+DEV unsigned wave_prefix(unsigned n, unsigned *total) {
+  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+  unsigned tot = 0, pre = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const unsigned long long bal = __ballot((n >> b) & 1u);
+    tot += (unsigned)__popcll(bal) << b;
+    pre += (unsigned)__popcll(bal & lt) << b;
+  }
+  *total = tot;
+  return pre;
+}
+
+// One atomic add of `total` for the whole wave, by its first active lane; returns the old value
+// to every lane.
+template <typename T>
+DEV T wave_reserve(T *counter, T total) {
+  const int leader = __ffsll((long long)__ballot(1)) - 1;
+  T base = 0;
+  if ((int)__lane_id() == leader) base = atomicAdd(counter, total);
+  return __shfl(base, leader);
+}
+
+// Queue the pairs (i, jg + t) whose bit t is set in dm for the binary64 re-bin, as record
+// indices (the re-bin kernel maps them to the caller's indices).  Called by every active lane of
+// the wave after a group of columns, outside the pair loop: a branch there (with its global atomic
+// and lane exchange) splits the loop body into pieces the compiler cannot overlap.  Entries go to
+// the workgroup's LDS queue, which moves to the global queue once per work item (queue_flush);
+// entries past its capacity go to the global queue directly.  Both reservations are one atomic
+// per wave: the global queue counter is shared by every wave of the device, and under that
+// contention a returning atomic costs thousands of cycles.
+DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int jg, int slice_id) {
+  if (__ballot(dm != 0u) == 0ull) return;  // the common case: nothing to queue in the wave
+  const unsigned n = (unsigned)__popc(dm);
+  unsigned tot;
+  const unsigned pre = wave_prefix(n, &tot);
+  if (tot == 0u) return;  // uniform
+  unsigned k0 = (unsigned)m.qcap;
+  if (m.qcap > 0) k0 = wave_reserve<unsigned>(m.qn, tot) + pre;
+  const unsigned cap = (unsigned)m.qcap;
+  const unsigned n_lds = k0 >= cap ? 0u : (n < cap - k0 ? n : cap - k0);
+  unsigned tot_o;
+  const unsigned pre_o = wave_prefix(n - n_lds, &tot_o);
+  unsigned long long g = 0;
+  if (tot_o) g = wave_reserve<unsigned long long>(P.pending_count, (unsigned long long)tot_o) + pre_o;
+  unsigned e = 0;
   while (dm) {
     const int t = __ffs((int)dm) - 1;
     dm &= dm - 1u;
-    const float *rj = (const float *)(col + (int64_t)(jg + t - col_j0) * R);
-    const int oj = __float_as_int(rj[Rec<G, D>::id]);
-    const int a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
-    if (k < (unsigned)m.qcap) {
-      m.q[k] = make_int2(a, b);
+    if (e < n_lds) {
+      m.q[k0 + e] = make_int2(i, jg + t);
     } else {
-      const unsigned long long g = atomicAdd(P.pending_count, 1ull);
-      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, a, b, 0};
+      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, 0};
+      ++g;
     }
-    ++k;
+    ++e;
   }
 }
 
@@ -1032,7 +1096,7 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
         dm |= 1u << (j - jg);
       ++j;
     }
-    queue_deferred<G, D>(P, m, dm, ri, col, col_j0, jg, slice_id);
+    queue_deferred(P, m, dm, i, jg, slice_id);
   }
   if (pa.h >= 0) {
     if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
@@ -1158,11 +1222,17 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
     for (unsigned long long k = beg + threadIdx.x; k < end; k += blockDim.x) {
       const Pending q = P.pending[k];
       const Slice s = P.slices[q.slice];
+      // queued as binary32 record indices (records may be permuted or spatially sorted): map them
+      // to the caller's indices, so that the pair is evaluated as the reference does, with i < j
+      constexpr int F = Rec<G, D>::f32;
+      const int ci = __float_as_int(P.rec32[(s.pt_off + q.i) * F + Rec<G, D>::id]);
+      const int cj = __float_as_int(P.rec32[(s.pt_off + q.j) * F + Rec<G, D>::id]);
+      const int64_t qi = ci < cj ? ci : cj, qj = ci < cj ? cj : ci;
       double A[S], Bv[S];
 #pragma unroll
       for (int t = 0; t < S; ++t) {
-        A[t] = P.rec64[(s.pt_off + q.i) * S + t];
-        Bv[t] = P.rec64[(s.pt_off + q.j) * S + t];
+        A[t] = P.rec64[(s.pt_off + qi) * S + t];
+        Bv[t] = P.rec64[(s.pt_off + qj) * S + t];
       }
       double v[2];
       const int c = exact_cell<G, D, J, SP ? (J ? 4 : 3) : 0>(P, s, A, Bv, P.cells + s.tab_off,
