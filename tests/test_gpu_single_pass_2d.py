@@ -200,3 +200,25 @@ def test_single_pass_2d_public_api(ctx):
             assert np.array_equal(res2[name].counts[..., b], cb[t])
     with pytest.raises(sfamd.DimensionMismatch):
         sfamd.calculate_structure_functions_single_pass_2d(x, u, dedges, (vl[0],) * 5 + (vl[0][:5],))
+
+
+def test_single_pass_2d_queue_growth(ctx):
+    """Log value bins reaching down to 1e-4 of the mean |du|^2 leave ~1 % of the sphere's plane
+    pairs undecidable in binary32: more than the re-bin queue's first capacity (2^22).  The call
+    grows the queue and reruns the binary32 kernel instead of redoing everything in binary64:
+    counts equal the oracle, and the sums are the binary32 path's (not bit-identical to the
+    binary64 kernel's)."""
+    from sfamd import synthetic
+    x, u, dedges, _ = synthetic.config4(N=12000)
+    m = 2.0 * float(np.sum(np.var(u, axis=1)))
+    pos = np.concatenate([[0.0], np.geomspace(1e-4 * m, 8.0 * m, 48)])
+    sgn = np.linspace(-4.0 * m ** 1.5, 4.0 * m ** 1.5, 49)
+    vl = [pos, pos, pos, sgn, sgn, sgn]
+    gs, gc = gpu_sp2d(ctx, x, u, dedges, vl, "deg", "mixed")
+    queued = ctx.stats()["rebinned"]
+    assert queued > (1 << 22), queued
+    os_, oc, oa, og = _oracle.single_pass_2d(x, u, dedges, vl, geometry="deg", nthreads=8, with_mag=True)
+    check2d(gs, gc, os_, oc, oa, og, "mixed", "queue growth")
+    s64, c64 = gpu_sp2d(ctx, x, u, dedges, vl, "deg", "fp64")
+    assert np.array_equal(c64, gc)
+    assert not np.array_equal(s64, gs)
