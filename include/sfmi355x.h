@@ -78,6 +78,19 @@ extern "C" {
                                  error <= 2^-23 of its value bin's width.  Other shapes run as
                                  SF_PREC_MIXED.                                                  */
 
+/* semantics: the reference evaluates a point field (PointField, x (W,N), u (D,N)) with its
+ * point-field kernels and fields with auxiliary axes (u (D,N,B...), shared or per-slice x) with its
+ * batch-leading kernels; on the flat geometry these differ in the last bits:
+ *   SF_SEM_POINT: distance binned as r^2 against squared edges (squared digitize plan,
+ *                 BinEdges.jl:410-558), values from _sf_raw (StructureFunctionTypes.jl:227-244),
+ *                 single-pass du_L = dot(du, dx) * inv(sqrt(r^2)) (serial_single_pass.jl:245-253);
+ *   SF_SEM_BATCH: distance r = sqrt(r^2) binned against the edges themselves, values from the
+ *                 operator on rhat = dx / r, single-pass du_L = dot(du, rhat) (pair_frame,
+ *                 pair_direction, pair_delta: HelperFunctions.jl:248-330; batch_leading.jl:105-330).
+ * The sphere is evaluated the same way by both. */
+#define SF_SEM_POINT 0
+#define SF_SEM_BATCH 1
+
 typedef struct sf_ctx sf_ctx;
 
 typedef struct {
@@ -98,6 +111,8 @@ typedef struct {
   int32_t precision;          /* SF_PREC_*                                                        */
   int32_t count_bits;         /* 32 or 64: width of the caller's count type (UInt32 default,
                                  dispatch.jl:69); calls whose N(N-1)/2 exceeds it are rejected    */
+  int32_t semantics;          /* SF_SEM_*: which of the reference's pair loops the results follow */
+  int32_t reserved;           /* 0                                                                */
 } sf_problem;
 
 /* Context: device, stream, persistent device workspace.

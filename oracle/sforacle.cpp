@@ -297,9 +297,21 @@ void row(const Prob &P, int64_t i, Acc &acc) {
         du[k] = p->u[k + D * j] - Ui[k];
       }
       const double r2 = sdot(dx, dx, D);
-      const double val = sf_raw(P.o, du, dx, r2);
-      // squared_bin(SquaredGeneralPlan) = searchsortedfirst(sqedges, r2) - 1
-      const int64_t b = count_isless(P.sqedges.data(), nb + 1, r2);
+      double val;
+      int64_t b;
+      if (p->batch_leading) {
+        // pair_frame / pair_direction / pair_delta (HelperFunctions.jl:248-330), as
+        // _bl_shared_1d! and _bl_joint2d_shared! use them (batch_leading.jl:105-215)
+        const double r = sqrt(r2);
+        double rh[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < D; ++k) rh[k] = dx[k] / r;
+        val = op_apply(P.o, du, rh);
+        b = count_isless(p->dist_edges, nb + 1, r);
+      } else {
+        val = sf_raw(P.o, du, dx, r2);
+        // squared_bin(SquaredGeneralPlan) = searchsortedfirst(sqedges, r2) - 1
+        b = count_isless(P.sqedges.data(), nb + 1, r2);
+      }
       if (1 <= b && b <= nb) {
         const double mag = mag_of(du, D, P.order);
         if (nv == 0) {
@@ -429,13 +441,20 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
     }
     const double r2 = sdot(dx, dx, D);
     *distance = sqrt(r2);
-    *value = sf_raw(o, du, dx, r2);
-    std::vector<double> sq(p->n_dist + 1);
-    for (int k = 0; k <= p->n_dist; ++k) {
-      const double e = p->dist_edges[k];
-      sq[k] = (e == -__builtin_inf()) ? e : e * e;
+    if (p->batch_leading) {
+      double rh[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < D; ++k) rh[k] = dx[k] / *distance;
+      *value = op_apply(o, du, rh);
+      *dbin = count_isless(p->dist_edges, p->n_dist + 1, *distance);
+    } else {
+      *value = sf_raw(o, du, dx, r2);
+      std::vector<double> sq(p->n_dist + 1);
+      for (int k = 0; k <= p->n_dist; ++k) {
+        const double e = p->dist_edges[k];
+        sq[k] = (e == -__builtin_inf()) ? e : e * e;
+      }
+      *dbin = count_isless(sq.data(), p->n_dist + 1, r2);
     }
-    *dbin = count_isless(sq.data(), p->n_dist + 1, r2);
     if (p->n_val > 0) *vbin = count_isless(p->val_edges, p->n_val + 1, *value);
     return 1;
   }
@@ -505,10 +524,19 @@ extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t 
             du[d] = q.u[d + D * j] - q.u[d + D * i];
           }
           const double r2 = sdot(dx, dx, D);
-          const double inv_r = 1.0 / sqrt(r2);
-          duL = sdot(du, dx, D) * inv_r;
+          if (q.batch_leading) {
+            // _bl_sp1d_shared! / _bl_sp2d_shared! (batch_leading.jl:270-330): du_L = dot(du, rhat)
+            const double r = sqrt(r2);
+            double rh[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < D; ++d) rh[d] = dx[d] / r;
+            duL = sdot(du, rh, D);
+            b = count_isless(q.dist_edges, nb + 1, r);
+          } else {
+            const double inv_r = 1.0 / sqrt(r2);
+            duL = sdot(du, dx, D) * inv_r;
+            b = count_isless(P.sqedges.data(), nb + 1, r2);
+          }
           dn2 = sdot(du, du, D);
-          b = count_isless(P.sqedges.data(), nb + 1, r2);
           if (!(1 <= b && b <= nb)) continue;
         } else {
           const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],
@@ -628,11 +656,19 @@ extern "C" int sfo_hist_single_pass_2d(const sfo_problem *p, int nthreads, int64
             du[d] = q.u[d + D * j] - q.u[d + D * i];
           }
           const double r2 = sdot(dx, dx, D);
-          const double inv_r = 1.0 / sqrt(r2);
-          duL = sdot(du, dx, D) * inv_r;
+          if (q.batch_leading) {
+            const double r = sqrt(r2);
+            double rh[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < D; ++d) rh[d] = dx[d] / r;
+            duL = sdot(du, rh, D);
+            b = count_isless(q.dist_edges, nb + 1, r);
+          } else {
+            const double inv_r = 1.0 / sqrt(r2);
+            duL = sdot(du, dx, D) * inv_r;
+            b = count_isless(P.sqedges.data(), nb + 1, r2);
+          }
           dn2 = sdot(du, du, D);
           scale = sqrt(dn2);
-          b = count_isless(P.sqedges.data(), nb + 1, r2);
           if (!(1 <= b && b <= nb)) continue;
         } else {
           const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],

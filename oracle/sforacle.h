@@ -41,6 +41,10 @@ typedef struct {
   const double *dist_edges;
   int32_t n_val;          /* 0 => 1D histogram; else value bins (joint histogram)      */
   const double *val_edges;
+  int32_t batch_leading;  /* flat pairs as the batch-leading kernels evaluate them      */
+  int32_t reserved;       /* (fields with auxiliary axes, batch_leading.jl:105-330):
+                             r = sqrt(dx.dx) binned against the edges, values from the
+                             operator on rhat = dx / r; 0: the point-field kernels     */
 } sfo_problem;
 
 /* Histogram over outer rows i = i_begin, i_begin + i_step, ... < i_end (0-based), pairs (i, j>i).

@@ -315,6 +315,8 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
     return ctx->fail(SF_E_ARG, "precision must be SF_PREC_FP64, SF_PREC_MIXED or SF_PREC_MIXED_FIXED");
   if (p->count_bits != 32 && p->count_bits != 64)
     return ctx->fail(SF_E_ARG, "count_bits must be 32 or 64");
+  if (p->semantics != SF_SEM_POINT && p->semantics != SF_SEM_BATCH)
+    return ctx->fail(SF_E_ARG, "semantics must be SF_SEM_POINT or SF_SEM_BATCH");
   const unsigned __int128 pairs = (unsigned __int128)p->N * (unsigned __int128)(p->N > 0 ? p->N - 1 : 0) / 2;
   const unsigned __int128 cmax = p->count_bits == 32 ? 0xffffffffull : 0xffffffffffffffffull;
   if (pairs > cmax)
@@ -339,6 +341,7 @@ int validate_batch(sf_ctx *ctx, int n, const sf_problem *s, bool host_ptrs) {
       const sf_problem &a = s[0], &b = s[k];
       if (a.D != b.D || a.W != b.W || a.geometry != b.geometry || a.op != b.op || a.nl != b.nl ||
           a.nt != b.nt || a.n_dist != b.n_dist || a.n_val != b.n_val || a.precision != b.precision ||
+          a.semantics != b.semantics ||
           (a.geometry == SF_GEOM_SPHERE_DEG && a.radius != b.radius))
         return ctx->fail(SF_E_ARG, "slice %d: all slices must share D, geometry, operator, bin counts and precision", k);
     }
@@ -362,9 +365,11 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   std::vector<long double> k64(nE);
   for (int k = 0; k < nE; ++k) {
     const double e = p.dist_edges[k];
-    // flat: squared edges (a leading -Inf, InfPaddedBinEdges' underflow edge, stays -Inf);
-    // sphere: the edges themselves, -0.0 -> +0.0 (keys are > 0)
-    T.dedges[k] = flat ? (e == -INFINITY ? e : e * e) : (e == 0.0 ? 0.0 : e);
+    // flat, point-field kernels: squared edges (a leading -Inf, InfPaddedBinEdges' underflow
+    // edge, stays -Inf); flat, batch-leading kernels: the edges themselves (r = sqrt(r^2) is
+    // binned); sphere: the edges themselves, -0.0 -> +0.0 (keys are > 0)
+    if (flat) T.dedges[k] = p.semantics == SF_SEM_BATCH ? e : (e == -INFINITY ? e : e * e);
+    else T.dedges[k] = e == 0.0 ? 0.0 : e;
     k64[k] = T.dedges[k];
   }
   Partition P64 = make_partition(k64, 1024, 6);
@@ -1080,6 +1085,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.nmom = nmom;
   P.fx = pl.fx ? 1 : 0;
   P.qcap = qcap;
+  P.bl = (flat && p0.semantics == SF_SEM_BATCH) ? 1 : 0;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
   return SF_OK;
 }

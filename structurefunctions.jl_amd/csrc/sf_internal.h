@@ -112,6 +112,8 @@ struct Params {
   int32_t fx;              // joint histogram kept as 64-bit fixed-point words (kFx* below)
   int32_t qcap;            // binary32 kernel: re-bin queue entries staged in LDS per workgroup
   int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
+  int32_t bl;              // flat pairs follow the batch-leading kernels (SF_SEM_BATCH)
+  int32_t pad3;
 };
 
 // Fixed-point joint cell: one 64-bit word, count in bits [44, 64), sum of q in bits [0, 44).

@@ -195,8 +195,9 @@ DEV int cell64(double key, const Slice &s) {
 DEV int dist_bin64(double key, const Cell *tab, const double *E, int nE, const Slice &s) {
   if (!(key > 0.0)) {
     if (key != key) return nE;
+    // Julia isless: -0.0 sorts below +0.0 (a flat distance of +0.0 lies above a -0.0 edge)
     int b = 0;
-    while (b < nE && E[b] < key) ++b;
+    while (b < nE && (E[b] < key || (E[b] == key && __builtin_signbit(E[b]) && !__builtin_signbit(key)))) ++b;
     return b;
   }
   int b = tab[cell64(key, s)].bx;
@@ -266,6 +267,40 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       du[k] = B[D + k] - A[D + k];
     }
     const double r2 = sdot<D>(dx, dx);
+    if (P.bl) {
+      // batch-leading kernels (batch_leading.jl:105-330): pair_frame's r = sqrt(dx.dx) binned
+      // against the edges themselves, rhat = dx / r (pair_direction), values from the operator
+      // on (du, rhat), single-pass du_L = dot(du, rhat) (HelperFunctions.jl:248-330)
+      const double r = __builtin_sqrt(r2);
+      db = dist_bin64(r, tab, dE, nEd, s);
+      if (db < 1 || db > P.n_dist) return -1;
+      double rh[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) rh[k] = dx[k] / r;
+      if constexpr (OPK == 3 || OPK == 4) {
+        const double duL = sdot<D>(du, rh);
+        const double dn2 = sdot<D>(du, du);
+        if constexpr (OPK == 3) {
+          val[0] = duL;
+          val[1] = dn2;
+          return db - 1;
+        } else {
+          v = sp_invariant(s.plane, duL, dn2);
+        }
+      } else if constexpr (OPK == 1) {
+        v = 1.0 * sdot<D>(du, rh);
+      } else if constexpr (OPK == 2) {
+        const double L = sdot<D>(du, rh);
+        v = 1.0 * (L * L);
+      } else {
+        v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+      }
+      *val = v;
+      if (!J) return db - 1;
+      const int vb = val_bin64(v, vE, P.n_val + 1, s);
+      if (vb < 1 || vb > P.n_val) return -1;
+      return (db - 1) + P.n_dist * (vb - 1);
+    }
     db = dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
     if constexpr (OPK == 3 || OPK == 4) {

@@ -67,6 +67,8 @@ struct SfProblem
     val_edges::Ptr{Float64}
     precision::Int32
     count_bits::Int32
+    semantics::Int32   # SF_SEM_POINT (point-field kernels) here; SF_SEM_BATCH for auxiliary axes
+    reserved::Int32
 end
 
 const _CONTEXTS = Dict{Int, Ptr{Cvoid}}()
@@ -128,7 +130,7 @@ function _run!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, 
                       Int32(length(de) - 1), pointer(de),
                       Int32(vbins === nothing ? 0 : length(ve) - 1),
                       vbins === nothing ? Ptr{Float64}(C_NULL) : pointer(ve),
-                      _prec(b), _count_bits(CT))
+                      _prec(b), _count_bits(CT), Int32(0), Int32(0))
         rc = ccall((:sf_hist, libsfmi355x), Cint,
                    (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
                    ctx, p, sums, counts)
@@ -247,7 +249,7 @@ function _run_sp2d!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt
         p = SfProblem(Int32(size(uc, 1)), Int32(size(xc, 1)), Int64(size(xc, 2)),
                       pointer(xc), pointer(uc), geom, radius, Int32(6), Int32(0), Int32(0),
                       Int32(length(de) - 1), pointer(de), Int32(n_val), pointer(ve),
-                      _prec(b), _count_bits(CT))
+                      _prec(b), _count_bits(CT), Int32(0), Int32(0))
         rc = ccall((:sf_hist, libsfmi355x), Cint,
                    (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
                    ctx, p, sums, counts)

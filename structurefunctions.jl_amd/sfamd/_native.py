@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__
 
 SF_OK, SF_E_ARG, SF_E_DIM, SF_E_COUNT, SF_E_DEVICE, SF_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_MIXED, PREC_MIXED_FIXED = 0, 1, 2
+SEM_POINT, SEM_BATCH = 0, 1
 OP_SINGLE_PASS = 6
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
@@ -35,6 +36,7 @@ class SfProblem(ctypes.Structure):
         ("n_dist", ctypes.c_int32), ("dist_edges", ctypes.c_void_p),
         ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
         ("precision", ctypes.c_int32), ("count_bits", ctypes.c_int32),
+        ("semantics", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

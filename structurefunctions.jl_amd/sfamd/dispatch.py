@@ -110,7 +110,7 @@ def _assert_counts_representable(count_dtype, n_points: int):
             f"{n_pairs} for N={n_points} (typemax = {tmax}); pass count_eltype=UInt64 or Int64.")
 
 
-def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep):
+def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep, semantics=_native.SEM_POINT):
     """Build one SfProblem for a (W, N) / (D, N) slice; keep references alive in `keep`."""
     geom, radius, _ = geometry_of(metric)
     xc = np.asfortranarray(x2, dtype=np.float64)
@@ -132,6 +132,7 @@ def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep):
         p.n_val, p.val_edges = len(vedges) - 1, vedges.ctypes.data
     p.precision = precision
     p.count_bits = 64
+    p.semantics = semantics
     return p
 
 
@@ -148,9 +149,15 @@ def _slices(x, u, kind):
     return [(xx[:, :, b], uu[:, :, b]) for b in range(B)], aux
 
 
-def _run(sf, slices, dedges_list, vedges_list, metric, backend):
+def _semantics(kind):
+    """Fields with auxiliary axes go through the reference's batch-leading kernels
+    (batch.jl:201-217, batch_leading.jl:105-330), point fields through its point-field kernels."""
+    return _native.SEM_POINT if kind == "point" else _native.SEM_BATCH
+
+
+def _run(sf, slices, dedges_list, vedges_list, metric, backend, semantics=_native.SEM_POINT):
     keep = []
-    problems = [_problem(sf, xs, us, de, ve, metric, backend.precision_code, keep)
+    problems = [_problem(sf, xs, us, de, ve, metric, backend.precision_code, keep, semantics)
                 for (xs, us), de, ve in zip(slices, dedges_list, vedges_list)]
     nd = len(dedges_list[0]) - 1
     sp = sf.code[0] == _native.OP_SINGLE_PASS
@@ -209,7 +216,8 @@ def calculate_structure_function(sf, x, u, distance_bins, value_bins=None, count
     de = as_edges(distance_bins)
     ve = None if value_bins is None else as_edges(value_bins)
     slices, aux = _slices(x, u, kind)
-    sums, counts, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend)
+    sums, counts, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend,
+                                _semantics(kind))
     B = len(slices)
     if ve is None:
         s = sums.reshape(B, nd).T.reshape((nd,) + aux, order="F")
@@ -242,7 +250,8 @@ def calculate_structure_function_inplace(sums, counts, sf, x, u, distance_bins, 
     slices, aux = _slices(x, u, kind)
     if tuple(np.shape(sums)) != want + aux or tuple(np.shape(counts)) != want + aux:
         raise DimensionMismatch(f"sums/counts must have shape {want + aux}; got {np.shape(sums)}, {np.shape(counts)}")
-    s, c, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend)
+    s, c, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend,
+                        _semantics(kind))
     B = len(slices)
     ncell = nd * max(nv, 1)
     s = np.stack([s[b * ncell:(b + 1) * ncell].reshape(want, order="F") for b in range(B)], axis=-1)

@@ -22,6 +22,7 @@ class Prob(ctypes.Structure):
         ("op", ctypes.c_int32), ("nl", ctypes.c_int32), ("nt", ctypes.c_int32),
         ("n_dist", ctypes.c_int32), ("dist_edges", ctypes.c_void_p),
         ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
+        ("batch_leading", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -57,7 +58,9 @@ def lib():
     return _lib
 
 
-def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):
+def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, batch_leading=False):
+    """batch_leading: evaluate flat pairs as the reference's batch-leading kernels (fields with
+    auxiliary axes) rather than its point-field kernels."""
     keep = []
     xc = np.asfortranarray(x, dtype=np.float64)
     uc = np.asfortranarray(u, dtype=np.float64)
@@ -76,14 +79,15 @@ def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):
         ve = np.ascontiguousarray(vedges, dtype=np.float64)
         keep.append(ve)
         p.n_val, p.val_edges = len(ve) - 1, ve.ctypes.data
+    p.batch_leading = 1 if batch_leading else 0
     return p, keep
 
 
 def hist(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, nthreads=1,
-         compensated=True, rows=None, with_mag=False):
+         compensated=True, rows=None, with_mag=False, batch_leading=False):
     """Oracle histogram: returns (sums, counts, abs_sums[, mag_sums]); joint results are
     (n_dist, n_val).  abs_sums = sum |value|, mag_sums = sum ||du||^order per bin."""
-    p, keep = make(x, u, edges, op, geometry, radius, vedges)
+    p, keep = make(x, u, edges, op, geometry, radius, vedges, batch_leading)
     nd = p.n_dist
     nv = p.n_val
     ncell = nd * max(nv, 1)
@@ -102,9 +106,9 @@ def hist(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, nt
     return tuple(out) if with_mag else tuple(out[:3])
 
 
-def single_pass(x, u, edges, geometry="flat", radius=1.0, nthreads=1, rows=None):
+def single_pass(x, u, edges, geometry="flat", radius=1.0, nthreads=1, rows=None, batch_leading=False):
     """Six invariants in one pass: (sums, counts), each (6, n_dist) rows S2, L2, T2, S3, L3, L1T2."""
-    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None)
+    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None, batch_leading)
     nd = p.n_dist
     s = np.zeros(6 * nd)
     c = np.zeros(6 * nd, dtype=np.uint64)
@@ -116,7 +120,7 @@ def single_pass(x, u, edges, geometry="flat", radius=1.0, nthreads=1, rows=None)
 
 
 def single_pass_2d(x, u, edges, vedges, geometry="flat", radius=1.0, nthreads=1, rows=None,
-                   with_mag=False):
+                   with_mag=False, batch_leading=False):
     """Single pass, 2D form: (sums, counts), each (6, n_dist, n_val); with_mag adds sum |value| and
     sum ||du||^order per cell.  vedges is one edge vector shared by the six invariants or a
     sequence of six (each at least n_val + 1 long)."""
@@ -130,7 +134,7 @@ def single_pass_2d(x, u, edges, vedges, geometry="flat", radius=1.0, nthreads=1,
         nv = len(v) - 1
         ve6 = np.ascontiguousarray(np.tile(v, 6))
     assert ve6.size == 6 * (nv + 1)
-    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None)
+    p, keep = make(x, u, edges, (1, 0, 0), geometry, radius, None, batch_leading)
     p.n_val = nv
     nd = p.n_dist
     s = np.zeros(6 * nd * nv)
@@ -149,8 +153,8 @@ def single_pass_2d(x, u, edges, vedges, geometry="flat", radius=1.0, nthreads=1,
     return out
 
 
-def pair(x, u, edges, i, j, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None):
-    p, keep = make(x, u, edges, op, geometry, radius, vedges)
+def pair(x, u, edges, i, j, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, batch_leading=False):
+    p, keep = make(x, u, edges, op, geometry, radius, vedges, batch_leading)
     d, v = ctypes.c_double(), ctypes.c_double()
     db, vb = ctypes.c_int64(), ctypes.c_int64()
     ok = lib().sfo_pair(ctypes.byref(p), i, j, ctypes.byref(d), ctypes.byref(v), ctypes.byref(db), ctypes.byref(vb))

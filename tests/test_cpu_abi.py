@@ -57,6 +57,7 @@ def test_check_valid():
     (dict(op=5, nl=4), _native.SF_E_UNSUPPORTED),
     (dict(precision=5), _native.SF_E_ARG),
     (dict(count_bits=16), _native.SF_E_ARG),
+    (dict(semantics=2), _native.SF_E_ARG),
 ])
 def test_check_rejects(kw, code):
     p, keep = prob(**kw)
@@ -119,3 +120,6 @@ def test_batch_mismatch_rejected():
     p1, k1 = prob()
     p2, k2 = prob(nl=3)
     assert _native.check_problems([p1, p2])[0] == _native.SF_E_ARG
+    p3, k3 = prob(semantics=_native.SEM_BATCH)
+    assert _native.check_problems([p3])[0] == 0
+    assert _native.check_problems([p1, p3])[0] == _native.SF_E_ARG
