@@ -68,12 +68,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device copy of 1 GiB (counter calibration for HBM traffic)")
-    ap.add_argument("--workload", choices=["sphere", "batch", "local", "single_pass_2d"], default="sphere",
+    ap.add_argument("--workload", choices=["sphere", "batch", "local", "single_pass_2d", "aux"], default="sphere",
                     help="sphere: the headline (N=262144 spherical joint); batch: 256 ragged 2D time "
                          "slices in one plan (hipGraph) against sequential binary64 launches; local: "
                          "the sphere field binned only up to 0.05 rad (spatial sort and tile culling "
                          "in one-shot calls, against a plan, which evaluates every pair); "
-                         "single_pass_2d: the six invariants' joint histograms of the sphere field")
+                         "single_pass_2d: the six invariants' joint histograms of the sphere field; "
+                         "aux: eight velocity fields on the sphere points (auxiliary axis, B-strips)")
     ap.add_argument("--report-errors", action="store_true",
                     help="print the per-order error of the binary32 kernels against binary64 instead")
     args = ap.parse_args()
@@ -89,6 +90,8 @@ def main():
         return bench_local(args)
     if args.workload == "single_pass_2d":
         return bench_single_pass_2d(args)
+    if args.workload == "aux":
+        return bench_aux(args)
 
     import numpy as np
     import torch
@@ -309,6 +312,82 @@ def bench_single_pass_2d(args):
            "counts_bit_exact": bool(np.array_equal(c32, c64)),
            "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
            "rebinned_pairs": int(st32["rebinned"]), "kernel_ms": round(st32["hist_ns"] * 1e-6, 3)}
+    print(json.dumps(out))
+
+
+def bench_aux(args):
+    """Eight velocity fields on the headline's points (first --n points, default 65536 here): an
+    auxiliary axis u (2, N, 8) with shared positions, du_L^2 over the 64 log distance bins.  The
+    B-strip launch (geometry once per pair, four fields per strip) against the same eight slices
+    given distinct position arrays (one pair pass per field) and against the binary64 kernel;
+    the rate counts field-pairs (8 N(N-1)/2 per call)."""
+    import numpy as np
+    import torch
+
+    from sfamd import _native, synthetic
+
+    n = args.n if args.n != 262144 else 65536
+    x, u0, dedges, _ = synthetic.config4(N=n)
+    N = x.shape[1]
+    B = 8
+    rng = np.random.default_rng(20261203)
+    lon, lat = np.radians(x[0]), np.radians(x[1])
+    Pt = np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)])
+    E = np.stack([-np.sin(lon), np.cos(lon), np.zeros_like(lon)])
+    Nn = np.stack([-np.sin(lat) * np.cos(lon), -np.sin(lat) * np.sin(lon), np.cos(lat)])
+    us = []
+    for b in range(B):
+        U = synthetic._flat_field(rng, Pt, 128, 1.0, 256.0, -5 / 6)
+        us.append(np.stack([np.sum(U * E, axis=0), np.sum(U * Nn, axis=0)]))
+    pairs = B * (N * (N - 1) // 2)
+    nd = len(dedges) - 1
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+    dxs = [torch.from_numpy(np.ascontiguousarray(x.T)).to(dev) for _ in range(B)]
+    dus = [torch.from_numpy(np.ascontiguousarray(v.T)).to(dev) for v in us]
+    ctx = _native.Context(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    s = torch.zeros(B * nd, dtype=torch.float64, device=dev)
+    c = torch.zeros(B * nd, dtype=torch.int64, device=dev)
+
+    def probs(shared, prec):
+        out = []
+        for b in range(B):
+            p = _device_problem(_native, (dx if shared else dxs[b]).data_ptr(), dus[b].data_ptr(), N, 2, 2, 1,
+                                (0, 2, 0), dedges, None, prec)
+            p.semantics = 1
+            out.append(p)
+        return out
+
+    def run(ps, reps):
+        best = None
+        for r in range(reps + 1):
+            s.zero_()
+            c.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.hist_device(ps, s.data_ptr(), c.data_ptr(), stream)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if r:
+                best = dt if best is None else min(best, dt)
+        return best, s.cpu().numpy(), c.cpu().numpy()
+
+    prec = "mixed" if args.precision == "mixed_fixed" else args.precision
+    t_strip, s1, c1 = run(probs(True, prec), max(1, args.steps))
+    items_strip = ctx.stats()["items"]
+    t_sep, s2, c2 = run(probs(False, prec), max(1, args.steps))
+    items_sep = ctx.stats()["items"]
+    t64, s3, c3 = run(probs(True, "fp64"), 1)
+    out = {"metric": "Gpairs/s (field-pairs: 8 fields x N(N-1)/2 per call)",
+           "value": round(pairs / t_strip / 1e9, 3), "unit": "Gpairs/s", "n_gpus": 1,
+           "ms_per_step": round(t_strip * 1e3, 3), "higher_is_better": True, "dtype": DTYPE[prec],
+           "data": "synthetic (eight seeded Fourier-mode fields on the sphere points)",
+           "config": {"workload": f"sphere N={N}, u (2, N, 8) shared positions, du_L^2 over 64 log distance bins",
+                      "pairs_per_step": pairs, "items_strip": items_strip, "items_per_slice": items_sep},
+           "per_slice_gpairs_s": round(pairs / t_sep / 1e9, 3), "speedup_from_strips": round(t_sep / t_strip, 3),
+           "fp64_gpairs_s": round(pairs / t64 / 1e9, 3),
+           "counts_equal_per_slice": bool(np.array_equal(c1, c2)), "counts_equal_fp64": bool(np.array_equal(c1, c3))}
     print(json.dumps(out))
 
 

@@ -44,6 +44,7 @@ hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b,
 int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, bool fx, int block,
                    size_t smem);
 int hist_opk(const Params &P);
+int strip_occupancy(int G, int D, int opk, int block, size_t smem);
 struct PrepSliceHost {
   const double *x, *u;
   int64_t N, pt_off;
@@ -168,6 +169,8 @@ struct Plan {
   int nout = 0;  // output slices (n is the internal slice count: six per slice for 2D single pass)
   bool J = false, mixed = false, sload = false, sp = false, fx = false;
   int block = 256, tj = 256, copies = 1, grid = 1, ncell_max = 1, n_items = 0;
+  int strip = 0;            // B-strip launch (fields per strip), 0: off
+  int n_items64 = 0;        // B-strip: the per-slice items the binary64 kernel runs instead
   int64_t total_pts = 0, max_n = 0, total_pairs = 0, cap = 0;
   Params P{};
   DevBuf<double> rec64;
@@ -181,6 +184,7 @@ struct Plan {
   DevBuf<unsigned long long> acc_c;
   DevBuf<int> counter;
   DevBuf<unsigned long long> pcount;
+  DevBuf<Item> items64;
   DevBuf<Pending> pending;
   DevBuf<PrepSliceHost> prep;
   DevBuf<int> flag;
@@ -196,6 +200,7 @@ struct Plan {
     rec64.release(); rec32.release(); slices.release(); items.release(); cells.release();
     dedges.release(); vedges.release(); vedges32.release(); acc_s.release(); acc_c.release();
     counter.release(); pcount.release(); pending.release(); prep.release(); flag.release();
+    items64.release();
     perm.release();
   }
 };
@@ -823,6 +828,17 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   pl.mixed = p0.precision != SF_PREC_FP64;
   pl.ncell = out_cells(p0);  // per input slice
   pl.sp = p0.op == SF_OP_SINGLE_PASS && !planes;
+  // B-strip: slices that share positions (the auxiliary axes of one field: same x, N, edges) and
+  // bin 1D histograms in binary32 evaluate each pair's geometry once for up to kStripMax fields
+  pl.strip = 0;
+  if (p0.precision != SF_PREC_FP64 && p0.n_val == 0 && p0.op != SF_OP_SINGLE_PASS && n_in >= 2) {
+    bool same = true;
+    for (int s = 1; s < n && same; ++s)
+      same = sl[s].x == p0.x && sl[s].N == p0.N && xmax[s] == xmax[0] &&
+             memcmp(sl[s].dist_edges, p0.dist_edges, sizeof(double) * (p0.n_dist + 1)) == 0;
+    if (same) pl.strip = kStripMax;
+  }
+  const int strip = pl.strip;
   pl.S64 = rec_d64(pl.G, pl.D);
   pl.F32 = rec_f32(pl.G, pl.D);
   const int G = pl.G, D = pl.D, ncell = pl.ncell, S64 = pl.S64, F32 = pl.F32;
@@ -863,7 +879,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     d.out_off = planes ? (int64_t)(s / kSpRows) * ncell + s % kSpRows : (int64_t)s * ncell;
     d.plane = planes ? s % kSpRows : 0;
     d.xerr = flat ? round_up_f(ldexpl((long double)xmax[s], -28)) : 0.0f;
-    d.pad1 = 0;
+    d.strip_n = (strip && s % strip == 0) ? std::min(strip, n - s) : 0;
     d.tab_off = (int32_t)cells.size();
     cells.insert(cells.end(), T.tab64.begin(), T.tab64.end());
     d.tab32_off = (int32_t)cells.size();
@@ -909,42 +925,46 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
   int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : 256);
   // column records staged per LDS chunk; the fixed-point joint launch keeps two workgroups per CU
-  const int tj = std::min(block, pl.fx ? 128 : 256);
+  const int tj = std::min(block, (pl.fx || strip) ? 128 : 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
   // (binary32 kernel only); 0 picks the default, measured at N = 65536: scalar loads for joint
   // histograms (sphere 212 vs 204 Gpairs/s; they also keep the record out of VGPRs, which the
   // two-workgroup fixed-point launch needs), LDS staging for 1D (sphere 432 vs 371)
-  const bool sload = mixed && (t_variant == 2 || (t_variant == 0 && J));
+  const bool sload = mixed && !strip && (t_variant == 2 || (t_variant == 0 && J));
   const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
-  const int stage_bytes = sload ? 0 : tj * rec_bytes;
+  // a B-strip stages every field's column records
+  const int stage_bytes = sload ? 0 : tj * rec_bytes * (strip ? strip : 1);
   int copies = 1;
   if (!J) {
     copies = 64;
-    while (copies > 1 && (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0).total * 2 > ctx->max_smem)
+    while (copies > 1 &&
+           (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0, strip).total * 2 > ctx->max_smem)
       copies >>= 1;
   }
-  const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0);
+  const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0, strip);
   const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom, 0, 0);
   if ((size_t)std::max(L0.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
                      ncell, std::max(L0.total, L64.total), ctx->max_smem);
-  int bpc = hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L0.total);
+  auto occupancy = [&](int blk, size_t bytes) {
+    return strip ? strip_occupancy(G, D, hist_opk(Pk), blk, bytes)
+                 : hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, blk, bytes);
+  };
+  int bpc = occupancy(block, (size_t)L0.total);
   if (bpc < 1) bpc = 1;
   if (!t_block && pl.fx && bpc * block < 1024) {
     // registers allow only one 768-thread workgroup per CU: use 1024 threads instead
     block = 1024;
-    bpc = std::max(1, hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)L0.total));
+    bpc = std::max(1, occupancy(block, (size_t)L0.total));
   }
   // binary32 kernel: the LDS re-bin queue, as large as fits without costing a workgroup per CU
   // (0: every queued pair takes the global queue's atomic)
   int qcap = 0;
   if (mixed) {
     for (int q = kQueueCap; q >= 64 && qcap == 0; q >>= 1) {
-      const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q).total;
-      if ((size_t)t <= ctx->max_smem &&
-          hist_occupancy(G, D, J, mixed, hist_opk(Pk), sload, pl.fx, block, (size_t)t) >= bpc)
-        qcap = q;
+      const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q, strip).total;
+      if ((size_t)t <= ctx->max_smem && occupancy(block, (size_t)t) >= bpc) qcap = q;
     }
   }
   if (t_bpc) bpc = std::min(bpc, t_bpc);
@@ -967,7 +987,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   std::vector<CullInfo> cinfo(n);
   std::vector<int32_t> perm_all;
   pl.culled = false;
-  if (cull && mixed) {
+  if (cull && mixed && !strip) {
     for (int s = 0; s < n; ++s) {
       const int rc = cull_prepare(ctx, sl[s], dx[s], xmax[s], block, st, cinfo[s]);
       if (rc) return rc;
@@ -993,7 +1013,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   std::vector<int> order(n);
   for (int s = 0; s < n; ++s) order[s] = s;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sl[a].N > sl[b].N; });
-  std::vector<Item> items;
+  std::vector<Item> items, items64;
   for (int s : order) {
     const int64_t N = sl[s].N;
     const CullInfo &ci = cinfo[s];
@@ -1009,11 +1029,15 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
         it.i0 = (int32_t)i0;
         it.j0 = (int32_t)j0;
         it.j1 = (int32_t)std::min<int64_t>(N, j0 + jc);
-        items.push_back(it);
+        // a B-strip's items belong to its leading slice
+        if (!strip || s % strip == 0) items.push_back(it);
+        if (strip) items64.push_back(it);
       }
     }
   }
   if (items.empty()) items.push_back(Item{0, 0, 0, 0});
+  if (strip && items64.empty()) items64.push_back(Item{0, 0, 0, 0});
+  pl.n_items64 = (int)items64.size();
   pl.n_items = (int)items.size();
   pl.grid = std::max(1, std::min(pl.n_items, max_blocks));
   pl.block = block;
@@ -1035,6 +1059,10 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   HIPCHK(pl.flag.ensure(4));
   HIPCHK(hipMemcpyAsync(pl.slices.p, sdesc.data(), sizeof(Slice) * n, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(pl.items.p, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice, st));
+  if (strip) {
+    HIPCHK(pl.items64.ensure(items64.size()));
+    HIPCHK(hipMemcpyAsync(pl.items64.p, items64.data(), sizeof(Item) * items64.size(), hipMemcpyHostToDevice, st));
+  }
   HIPCHK(hipMemcpyAsync(pl.cells.p, cells.data(), sizeof(Cell) * cells.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(pl.dedges.p, dedges.data(), sizeof(double) * dedges.size(), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(pl.vedges.p, vedges.data(), sizeof(double) * vedges.size(), hipMemcpyHostToDevice, st));
@@ -1086,6 +1114,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.fx = pl.fx ? 1 : 0;
   P.qcap = qcap;
   P.bl = (flat && p0.semantics == SF_SEM_BATCH) ? 1 : 0;
+  P.strip = strip;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
   return SF_OK;
 }
@@ -1113,8 +1142,16 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
     P.fx = 0;
     P.qcap = 0;
     P.flush_limit = 0x7fffffffLL;
+    if (pl.strip) {  // one slice at a time
+      P.strip = 0;
+      P.items = pl.items64.p;
+      P.n_items = pl.n_items64;
+    }
   }
-  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage, P.copies, P.nmom, P.fx, P.qcap);
+  const int stage_mixed = stage * ((use_mixed && pl.strip) ? pl.strip : 1);
+  P.stage_bytes = stage_mixed;
+  const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_mixed, P.copies, P.nmom, P.fx, P.qcap,
+                                   P.strip);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));

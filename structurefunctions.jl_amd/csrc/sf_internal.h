@@ -67,7 +67,7 @@ struct Slice {
   float vwf, vsf;
   int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
   float xerr;         // flat: 2^-28 max|x| (binary32 value bound: reference's direction error)
-  int32_t pad1;
+  int32_t strip_n;    // B-strip launch: fields of the strip this slice leads (0: not a leader)
 };
 
 struct Item {
@@ -77,7 +77,8 @@ struct Item {
 };
 
 struct Pending {      // a pair the binary32 kernel could not bin with certainty (binary32 record indices)
-  int32_t slice, i, j, pad;
+  int32_t slice, i, j;
+  int32_t fields;     // B-strip: slices slice .. slice + fields - 1 share the pair (0: one slice)
 };
 
 struct Params {
@@ -113,7 +114,7 @@ struct Params {
   int32_t qcap;            // binary32 kernel: re-bin queue entries staged in LDS per workgroup
   int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
   int32_t bl;              // flat pairs follow the batch-leading kernels (SF_SEM_BATCH)
-  int32_t pad3;
+  int32_t strip;           // B-strip launch: LDS histograms per workgroup (fields per strip), 0: off
 };
 
 // Fixed-point joint cell: one 64-bit word, count in bits [44, 64), sum of q in bits [0, 44).
@@ -149,14 +150,18 @@ inline constexpr int kQueueCap = 1024;
 // largest global re-bin queue (entries of 16 bytes: 4 GiB); a call needing more is redone with
 // the binary64 kernel
 inline constexpr int64_t kQueueMax = 1LL << 28;
+// B-strip launches: velocity fields evaluated per pair geometry (rows keep each field's
+// velocity record in registers)
+inline constexpr int kStripMax = 4;
 
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
                                                   int stage_bytes, int copies, int nmom, int fx,
-                                                  int qcap) {
+                                                  int qcap, int strip = 0) {
   SmemLayout L;
   // + kDummySlots: one scratch slot per lane, target of unconditional atomics of lanes that have
-  // nothing to add (never flushed)
-  const int nh = ((n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies) + kDummySlots;
+  // nothing to add (never flushed); a B-strip launch keeps one histogram per field
+  const int nh = ((n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies) * (strip > 1 ? strip : 1) +
+                 kDummySlots;
   int o = 0;
   L.hs = o; o += 8 * nh * nmom;
   L.hc = o; o += fx ? 0 : 4 * ((nh + 3) & ~3);  // fixed-point cells carry their count

@@ -471,7 +471,7 @@ struct Smem {
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx,
-                                   P.qcap);
+                                   P.qcap, P.strip);
   Smem m;
   m.nmom = P.nmom;
   m.fx = P.fx != 0;
@@ -481,7 +481,8 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.nd = P.n_dist;
   m.vstride = P.n_val > 0 ? joint_stride(P.n_val) : 0;
   m.nslots = P.n_val > 0 ? P.n_dist * m.vstride : P.n_dist * P.copies;
-  m.mstride = m.nslots + kDummySlots;
+  // B-strip: field b's histogram at slots [b nslots, (b + 1) nslots), scratch slots after all
+  m.mstride = m.nslots * (P.strip > 1 ? P.strip : 1) + kDummySlots;
   m.hs = (double *)(base + L.hs);
   m.hq = (unsigned long long *)(base + L.hs);
   m.hc = (unsigned *)(base + L.hc);
@@ -1054,7 +1055,8 @@ DEV T wave_reserve(T *counter, T total) {
 // entries past its capacity go to the global queue directly.  Both reservations are one atomic
 // per wave: the global queue counter is shared by every wave of the device, and under that
 // contention a returning atomic costs thousands of cycles.
-DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int jg, int slice_id) {
+DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int jg, int slice_id,
+                        int fields = 0) {
   if (__ballot(dm != 0u) == 0ull) return;  // the common case: nothing to queue in the wave
   const unsigned n = (unsigned)__popc(dm);
   unsigned tot;
@@ -1075,7 +1077,7 @@ DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int 
     if (e < n_lds) {
       m.q[k0 + e] = make_int2(i, jg + t);
     } else {
-      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, 0};
+      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, fields};
       ++g;
     }
     ++e;
@@ -1083,7 +1085,8 @@ DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int 
 }
 
 // Move the workgroup's LDS queue to the global queue (all threads; between barriers).
-DEV void queue_flush(const Params &P, const Smem &m, int slice_id, unsigned long long *s_base) {
+DEV void queue_flush(const Params &P, const Smem &m, int slice_id, unsigned long long *s_base,
+                     int fields = 0) {
   if (m.qcap == 0) return;
   const unsigned n0 = *m.qn;
   const unsigned n = n0 < (unsigned)m.qcap ? n0 : (unsigned)m.qcap;
@@ -1095,7 +1098,7 @@ DEV void queue_flush(const Params &P, const Smem &m, int slice_id, unsigned long
     const unsigned long long idx = base + k;
     if (idx < (unsigned long long)P.pending_cap) {
       const int2 e = m.q[k];
-      P.pending[idx] = Pending{slice_id, e.x, e.y, 0};
+      P.pending[idx] = Pending{slice_id, e.x, e.y, fields};
     }
   }
   __syncthreads();
@@ -1228,6 +1231,263 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
 }
 
 // ---------------------------------------------------------------------------------------------
+// hist32s: B-strip of fields sharing positions (auxiliary axes, batch_leading.jl:105-140): the
+// pair's geometry -- distance key, certified bin, frame -- is evaluated once and each of the
+// strip's velocity fields adds its value to its own LDS histogram (1D histograms).  Slices
+// s0 .. s0 + nf - 1 are the fields; their binary32 records are in one order (same N, same
+// positions), so column j of every field is record j of its slice.  Columns are staged through
+// LDS, field-major.  An undecidable pair (distance within its margin, degenerate, or a
+// non-finite value in any field) is queued once for all fields.
+// ---------------------------------------------------------------------------------------------
+template <int G, int D>
+struct StripRec {
+  // the strip's LDS stage per column: positions (flat x_hi, x_lo; sphere P_hi, P_lo) and per field
+  // the velocity (flat u_hi, u_lo; sphere U_hi, U_lo, u_r hi/lo), each padded to whole float4s
+  static constexpr int pos = (G == kFlat) ? 2 * D : 6;      // floats of the position part
+  static constexpr int PS = (pos + 3) / 4 * 4;
+  static constexpr int voff = (G == kFlat) ? 2 * D : 6;     // velocity part's offset in a record
+  static constexpr int vel = (G == kFlat) ? 2 * D : (D == 3 ? 8 : 6);
+  static constexpr int VS = (vel + 3) / 4 * 4;
+};
+
+template <int N4>
+DEV void lds_read4(const float *src, float *dst) {
+#pragma unroll
+  for (int k = 0; k < N4 / 4; ++k) {
+    const float4 q = ((const float4 *)src)[k];
+    dst[4 * k] = q.x; dst[4 * k + 1] = q.y; dst[4 * k + 2] = q.z; dst[4 * k + 3] = q.w;
+  }
+}
+
+template <int G, int D, int OPK, bool MASK>
+DEV void run_chunk_strip(const Params &P, const Slice &s, const Smem &m, const float *ri,
+                         const float (*rv)[StripRec<G, D>::VS], int nf, int i, int jb, int cnt,
+                         int slice_id) {
+  using SR = StripRec<G, D>;
+  constexpr bool needT = (OPK == 0);
+  constexpr bool needN2 = (OPK == 0);
+  const float *spos = (const float *)m.stage;
+  const float *svel = spos + (size_t)P.tj * SR::PS;
+  const int tj = P.tj;
+  const int nb = P.n_dist;
+  const int slot0 = (int)(__lane_id() & (m.copies - 1));
+  for (int jg = 0; jg < cnt; jg += 32) {
+    const int je = min(cnt, jg + 32);
+    unsigned dm = 0u;
+    for (int jj = jg; jj < je; ++jj) {
+      const int j = jb + jj;
+      const bool upper = !MASK || j > i;
+      float rj[SR::PS];
+      lds_read4<SR::PS>(spos + jj * SR::PS, rj);
+      float key, rinv = 0.f, inv_s = 0.f;
+      float dx[3], d[3], w[3];
+      if constexpr (G == kFlat) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) dx[k] = (rj[k] - ri[k]) + (rj[D + k] - ri[D + k]);
+        key = dx[0] * dx[0];
+#pragma unroll
+        for (int k = 1; k < D; ++k) key = __builtin_fmaf(dx[k], dx[k], key);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) d[k] = (rj[k] - ri[k]) + (rj[3 + k] - ri[3 + k]);
+        key = __builtin_fmaf(d[2], d[2], __builtin_fmaf(d[1], d[1], d[0] * d[0]));
+      }
+      const int c = cell32(key, s);
+      const float2 lh = m.lohi[c];
+      const int blo = m.blo[c];
+      const bool amb_d = (key >= lh.x) & (key <= lh.y);
+      const int db = (key > lh.y) ? blo + 1 : blo;
+      const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
+      bool amb = amb_d;
+      if constexpr (G == kFlat) {
+        rinv = frsq(key);
+        amb = amb || !(key > 1e-30f);
+      } else {
+        w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
+        w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
+        w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
+        const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
+        amb = amb || !(s2 > 1e-30f);
+        inv_s = frsq(s2);
+      }
+      float vals[kStripMax];
+#pragma unroll
+      for (int b = 0; b < kStripMax; ++b) {
+        vals[b] = 0.f;
+        if (b < nf) {
+          float rjb[SR::VS];
+          lds_read4<SR::VS>(svel + ((size_t)b * tj + jj) * SR::VS, rjb);
+          const float *rib = rv[b];
+          float L, T = 0.f, tn2 = 0.f, n2 = 0.f;
+          if constexpr (G == kFlat) {
+            float du[3];
+#pragma unroll
+            for (int k = 0; k < D; ++k) du[k] = (rjb[k] - rib[k]) + (rjb[D + k] - rib[D + k]);
+            float p = du[0] * dx[0];
+#pragma unroll
+            for (int k = 1; k < D; ++k) p = __builtin_fmaf(du[k], dx[k], p);
+            L = p * rinv;
+            if constexpr (needN2) {
+              n2 = du[0] * du[0];
+#pragma unroll
+              for (int k = 1; k < D; ++k) n2 = __builtin_fmaf(du[k], du[k], n2);
+            }
+            if constexpr (needT) {
+              const float cz = __builtin_fmaf(du[1], dx[0], -du[0] * dx[1]);
+              if constexpr (D == 2) {
+                T = cz * rinv;
+                tn2 = T * T;
+              } else {
+                T = cz * frsq(__builtin_fmaf(dx[1], dx[1], dx[0] * dx[0]));
+                const float cx = __builtin_fmaf(du[2], dx[1], -du[1] * dx[2]);
+                const float cy = __builtin_fmaf(du[0], dx[2], -du[2] * dx[0]);
+                tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
+              }
+            }
+          } else {
+            float dU[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dU[k] = (rjb[k] - rib[k]) + (rjb[3 + k] - rib[3 + k]);
+            L = __builtin_fmaf(d[2], dU[2], __builtin_fmaf(d[1], dU[1], d[0] * dU[0])) * inv_s;
+            if constexpr (needT) {
+              T = __builtin_fmaf(w[2], dU[2], __builtin_fmaf(w[1], dU[1], w[0] * dU[0])) * inv_s;
+              tn2 = T * T;
+              if constexpr (D == 3) {
+                const float dw = (rjb[6] - rib[6]) + (rjb[7] - rib[7]);
+                tn2 = __builtin_fmaf(dw, dw, tn2);
+              }
+              n2 = __builtin_fmaf(L, L, tn2);
+            }
+          }
+          const float v = op32<OPK>(P, D, L, T, tn2, n2);
+          amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
+          vals[b] = v;
+        }
+      }
+      const bool add = upper & !amb & db_ok;
+      const bool defer = upper & amb & (amb_d | db_ok);
+      if (add) {
+        // one count for the pair (every field of the strip shares the bin); one sum per field
+        const int h = (db - 1) * m.copies + slot0;
+        atomicAdd(&m.hc[h], 1u);
+#pragma unroll
+        for (int b = 0; b < kStripMax; ++b)
+          if (b < nf) atomicAdd(&m.hs[b * m.nslots + h], (double)vals[b]);
+      }
+      if (defer) dm |= 1u << (jj - jg);
+    }
+    queue_deferred(P, m, dm, i, jb + jg, slice_id, nf);
+  }
+}
+
+// flush a strip's histograms: the shared counts and each field's sums into the field's slice
+DEV void flush_strip(const Smem &m, const Params &P, int s0, int nf) {
+  for (int c = threadIdx.x; c < m.nd; c += blockDim.x) {
+    unsigned long long n = 0;
+    for (int k = 0; k < m.copies; ++k) {
+      n += m.hc[c * m.copies + k];
+      m.hc[c * m.copies + k] = 0u;
+    }
+    for (int b = 0; b < nf; ++b) {
+      double v = 0.0;
+      for (int k = 0; k < m.copies; ++k) {
+        const int h = b * m.nslots + c * m.copies + k;
+        v += m.hs[h];
+        m.hs[h] = 0.0;
+      }
+      if (n) {
+        const Slice sb = P.slices[s0 + b];
+        atomicAdd(&P.counts[sb.out_off + c], n);
+        atomicAdd(&P.sums[sb.out_off + c], v);
+      }
+    }
+  }
+}
+
+template <int G, int D, int OPK>
+__global__ void __launch_bounds__(1024) hist32s(Params P, const float *__restrict__ rec) {
+  extern __shared__ __align__(16) char smem_raw[];
+  using SR = StripRec<G, D>;
+  constexpr int F = Rec<G, D>::f32;
+  const Smem m = carve(smem_raw, P, P.stage_bytes);
+  __shared__ int s_item;
+  __shared__ unsigned long long s_qbase;
+  const int B = blockDim.x;
+  int cur = -1, nf = 1;
+  Slice s{};
+  unsigned long long since_flush = 0;
+  zero_hist(m);
+  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= P.n_items) break;
+    const Item w = P.items[it];
+    const unsigned long long work = (unsigned long long)B * (unsigned long long)(w.j1 - w.j0);
+    if (w.slice != cur || since_flush + work > (unsigned long long)P.flush_limit) {
+      if (cur >= 0) {
+        flush_strip(m, P, cur, nf);
+        __syncthreads();
+      }
+      if (w.slice != cur) {
+        cur = w.slice;
+        s = P.slices[cur];
+        nf = s.strip_n > 1 ? s.strip_n : 1;
+        load_tables(m, P, s, true);
+      }
+      since_flush = 0;
+    }
+    since_flush += work;
+    const int i = w.i0 + (int)threadIdx.x;
+    const bool iv = i < s.N;
+    float ri[SR::pos];
+#pragma unroll
+    for (int k = 0; k < SR::pos; ++k) ri[k] = iv ? rec[(s.pt_off + i) * F + k] : 0.0f;
+    float rv[kStripMax][SR::VS];
+#pragma unroll
+    for (int b = 0; b < kStripMax; ++b) {
+#pragma unroll
+      for (int k = 0; k < SR::VS; ++k) {
+        rv[b][k] = 0.f;
+        if (iv && b < nf && k < SR::vel) rv[b][k] = rec[(P.slices[cur + b].pt_off + i) * F + SR::voff + k];
+      }
+    }
+    for (int jb = w.j0; jb < w.j1; jb += P.tj) {
+      const int cnt = min(P.tj, w.j1 - jb);
+      __syncthreads();
+      // stage: positions of the columns, then each field's velocities (compact, float4-aligned)
+      float *spos = (float *)m.stage;
+      float *svel = spos + (size_t)P.tj * SR::PS;
+      for (int t = threadIdx.x; t < cnt * (1 + nf); t += B) {
+        const int jj = t % cnt, part = t / cnt;
+        if (part == 0) {
+          const float *src = rec + (s.pt_off + jb + jj) * F;
+#pragma unroll
+          for (int k = 0; k < SR::PS; ++k) spos[jj * SR::PS + k] = k < SR::pos ? src[k] : 0.f;
+        } else {
+          const int b = part - 1;
+          const float *src = rec + (P.slices[cur + b].pt_off + jb + jj) * F + SR::voff;
+          float *dst = svel + ((size_t)b * P.tj + jj) * SR::VS;
+#pragma unroll
+          for (int k = 0; k < SR::VS; ++k) dst[k] = k < SR::vel ? src[k] : 0.f;
+        }
+      }
+      __syncthreads();
+      if (iv) {
+        if (jb < w.i0 + B) run_chunk_strip<G, D, OPK, true>(P, s, m, ri, rv, nf, i, jb, cnt, cur);
+        else run_chunk_strip<G, D, OPK, false>(P, s, m, ri, rv, nf, i, jb, cnt, cur);
+      }
+    }
+    __syncthreads();
+    queue_flush(P, m, cur, &s_qbase, nf);
+  }
+  __syncthreads();
+  if (cur >= 0) flush_strip(m, P, cur, nf);
+}
+
+// ---------------------------------------------------------------------------------------------
 // recheck: binary64 binning of queued pairs (global-memory tables, global atomics)
 // ---------------------------------------------------------------------------------------------
 // SP: the single pass; with J, one plane of its 2D form (output cells every sixth)
@@ -1255,7 +1515,12 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
   if (beg < end) {
     const int home = P.pending[beg].slice;
     for (unsigned long long k = beg + threadIdx.x; k < end; k += blockDim.x) {
-      const Pending q = P.pending[k];
+      const Pending q0 = P.pending[k];
+      // a B-strip entry stands for the pair in each of its fields' slices
+      const int nfields = q0.fields > 1 ? q0.fields : 1;
+      for (int f = 0; f < nfields; ++f) {
+      Pending q = q0;
+      q.slice += f;
       const Slice s = P.slices[q.slice];
       // queued as binary32 record indices (records may be permuted or spatially sorted): map them
       // to the caller's indices, so that the pair is evaluated as the reference does, with i < j
@@ -1293,6 +1558,7 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
 #pragma unroll
           for (int t = 0; t < NM; ++t) atomicAdd(&P.sums[o + (SP1 ? sp_row(t) : 0)], mv[t]);
         }
+      }
       }
     }
     __syncthreads();
@@ -1467,9 +1733,30 @@ int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, bool f
 
 int hist_opk(const Params &P) { return opk_of(P); }
 
+template <int G, int D>
+static const void *strip_fn(int opk) {
+  if (opk == 1) return (const void *)&hist32s<G, D, 1>;
+  if (opk == 2) return (const void *)&hist32s<G, D, 2>;
+  return (const void *)&hist32s<G, D, 0>;
+}
+
+static const void *pick_strip(int G, int D, int opk) {
+  if (G == kFlat) return D == 2 ? strip_fn<kFlat, 2>(opk) : strip_fn<kFlat, 3>(opk);
+  return D == 2 ? strip_fn<kSphere, 2>(opk) : strip_fn<kSphere, 3>(opk);
+}
+
+int strip_occupancy(int G, int D, int opk, int block, size_t smem) {
+  const void *f = pick_strip(G, D, opk);
+  (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, smem) != hipSuccess) return 1;
+  return nb;
+}
+
 hipError_t launch_hist(int G, int D, bool J, bool mixed, bool sload, const Params &P, int grid,
                        int block, size_t smem, hipStream_t st) {
-  const void *f = pick_hist(G, D, J, mixed, opk_of(P), sload, mixed && P.fx);
+  const void *f = (mixed && P.strip > 1) ? pick_strip(G, D, opk_of(P))
+                                         : pick_hist(G, D, J, mixed, opk_of(P), sload, mixed && P.fx);
   (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   Params Pc = P;
   const void *rec = mixed ? (const void *)P.rec32 : (const void *)P.rec64;
