@@ -5,7 +5,8 @@
 #   _dispatch_execution_backend(backend, shape::PointField{D}, sf, x, u, distance_bins, value_bins; ...)
 #   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins; ...)
 #   _dispatch_execution_backend!(backend, shape::PointField{D}, sums, counts, sf, x, u, distance_bins, value_bins; ...)
-# (src/Calculations/dispatch.jl:377-477, 510-594), and the two single-pass hooks
+# (src/Calculations/dispatch.jl:377-477, 510-594), the same four for fields with auxiliary axes
+# (shape::SharedPositionField / VaryingPositionField, batch-leading semantics), and the single-pass hooks
 #   _dispatch_single_pass(backend, shape::PointField, x, u, distance_bins; ...)
 #   _dispatch_single_pass!(backend, sums, counts, x, u, distance_bins; ...)
 # (src/Calculations/serial_single_pass.jl:400-535), and the 2D single-pass hooks
@@ -187,6 +188,97 @@ function SFC._dispatch_execution_backend!(
     s = zeros(Float64, length(sums))
     c = zeros(UInt64, length(counts))
     _run!(b, s, c, sf, x, u, distance_bins, value_bins, distance_metric, eltype(counts))
+    sums .+= reshape(s, size(sums))
+    counts .+= reshape(c, size(counts))
+    return nothing
+end
+
+# ---- auxiliary axes: u (D, N, B...) over shared x (W, N) or per-slice x (W, N, B...) -----------
+# The reference runs these through its batch-leading kernels (batch.jl:201-217,
+# batch_leading.jl:105-330), whose flat pairs bin r = sqrt(dx.dx) against the edges: the library
+# follows them with semantics = SF_SEM_BATCH.  Slices share x and the edges, so the library's
+# B-strips evaluate each pair's geometry once for several fields.  Outputs are (n_bins[, n_val],
+# aux...), column-major, like the reference's batch results.
+function _run_aux!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
+                   dbins, vbins, metric, CT)
+    ctx = _context(b.device)
+    D, N = size(u, 1), size(u, 2)
+    B = prod(size(u)[3:end])
+    uc = reshape(convert(Array{Float64}, u), D, N, B)
+    shared = ndims(x) == 2
+    xc = shared ? convert(Matrix{Float64}, x) : reshape(convert(Array{Float64}, x), size(x, 1), N, B)
+    W = size(xc, 1)
+    de = _edges(dbins)
+    ve = vbins === nothing ? Float64[] : _edges(vbins)
+    op, nl, nt = _opcode(sf)
+    geom, radius = _geometry(metric)
+    GC.@preserve xc uc de ve sums counts begin
+        probs = [SfProblem(Int32(D), Int32(W), Int64(N),
+                           shared ? pointer(xc) : pointer(xc, 1 + (k - 1) * W * N),
+                           pointer(uc, 1 + (k - 1) * D * N), geom, radius, op, nl, nt,
+                           Int32(length(de) - 1), pointer(de),
+                           Int32(vbins === nothing ? 0 : length(ve) - 1),
+                           vbins === nothing ? Ptr{Float64}(C_NULL) : pointer(ve),
+                           _prec(b), _count_bits(CT), Int32(1), Int32(0)) for k in 1:B]
+        rc = ccall((:sf_hist_batch, libsfmi355x), Cint,
+                   (Ptr{Cvoid}, Int32, Ptr{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
+                   ctx, Int32(B), probs, sums, counts)
+        _check(ctx, rc)
+    end
+    return nothing
+end
+
+const _AuxShape = Union{SFC.SharedPositionField, SFC.VaryingPositionField}
+
+function SFC._dispatch_execution_backend(
+    b::MI355XBackend, ::_AuxShape, sf::SFT.AbstractPairwiseStructureFunctionType,
+    x, u, distance_bins, count_eltype::Type{CT} = UInt32;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+) where {CT}
+    nb = length(distance_bins) - 1
+    aux = size(u)[3:end]
+    sums = zeros(Float64, nb * prod(aux))
+    counts = zeros(UInt64, nb * prod(aux))
+    _run_aux!(b, sums, counts, sf, x, u, distance_bins, nothing, distance_metric, CT)
+    return SFO.StructureFunctionSumsAndCounts(sf, distance_bins, reshape(sums, nb, aux...),
+        CT.(reshape(counts, nb, aux...)))
+end
+
+function SFC._dispatch_execution_backend(
+    b::MI355XBackend, ::_AuxShape, sf::SFT.AbstractPairwiseStructureFunctionType,
+    x, u, distance_bins, value_bins::AbstractVector;
+    distance_metric::DI.PreMetric = DI.Euclidean(), count_eltype::Type{CT} = UInt32, kwargs...,
+) where {CT}
+    nd, nv = length(distance_bins) - 1, length(value_bins) - 1
+    aux = size(u)[3:end]
+    sums = zeros(Float64, nd * nv * prod(aux))
+    counts = zeros(UInt64, nd * nv * prod(aux))
+    _run_aux!(b, sums, counts, sf, x, u, distance_bins, value_bins, distance_metric, CT)
+    return SFO.StructureFunction2DSumsAndCounts(sf, distance_bins, value_bins,
+        reshape(sums, nd, nv, aux...), CT.(reshape(counts, nd, nv, aux...)))
+end
+
+function SFC._dispatch_execution_backend!(
+    b::MI355XBackend, ::_AuxShape, sums::AbstractArray, counts::AbstractArray,
+    sf::SFT.AbstractPairwiseStructureFunctionType, x, u, distance_bins;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+)
+    s = zeros(Float64, length(sums))
+    c = zeros(UInt64, length(counts))
+    _run_aux!(b, s, c, sf, x, u, distance_bins, nothing, distance_metric, eltype(counts))
+    sums .+= reshape(s, size(sums))
+    counts .+= reshape(c, size(counts))
+    return nothing
+end
+
+function SFC._dispatch_execution_backend!(
+    b::MI355XBackend, ::_AuxShape, sums::AbstractArray, counts::AbstractArray,
+    sf::SFT.AbstractPairwiseStructureFunctionType, x, u, distance_bins, value_bins::AbstractVector;
+    distance_metric::DI.PreMetric = DI.Euclidean(), kwargs...,
+)
+    s = zeros(Float64, length(sums))
+    c = zeros(UInt64, length(counts))
+    _run_aux!(b, s, c, sf, x, u, distance_bins, value_bins, distance_metric, eltype(counts))
     sums .+= reshape(s, size(sums))
     counts .+= reshape(c, size(counts))
     return nothing
