@@ -1113,6 +1113,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.nmom = nmom;
   P.fx = pl.fx ? 1 : 0;
   P.qcap = qcap;
+  P.qw = qcap / (block / 64);
   P.bl = (flat && p0.semantics == SF_SEM_BATCH) ? 1 : 0;
   P.strip = strip;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
@@ -1141,6 +1142,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts, no queue
     P.fx = 0;
     P.qcap = 0;
+    P.qw = 0;
     P.flush_limit = 0x7fffffffLL;
     if (pl.strip) {  // one slice at a time
       P.strip = 0;

@@ -112,6 +112,7 @@ struct Params {
   int32_t nmom;            // accumulated moments per bin: 1, or 4 for the single-pass invariants
   int32_t fx;              // joint histogram kept as 64-bit fixed-point words (kFx* below)
   int32_t qcap;            // binary32 kernel: re-bin queue entries staged in LDS per workgroup
+  int32_t qw;              // ... of which each wave owns qcap / (block_rows / 64)
   int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
   int32_t bl;              // flat pairs follow the batch-leading kernels (SF_SEM_BATCH)
   int32_t strip;           // B-strip launch: LDS histograms per workgroup (fields per strip), 0: off
@@ -144,9 +145,11 @@ inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
 inline __host__ __device__ bool sp2d(int op, int n_val) { return op == 6 && n_val > 0; }
 
 inline constexpr int kDummySlots = 64;
-// largest re-bin queue staged in LDS per workgroup (entries); the host shrinks it where the LDS
-// it takes would cost a workgroup per CU, and entries past it go straight to the global queue
-inline constexpr int kQueueCap = 1024;
+// largest re-bin queue staged in LDS per workgroup (entries, split evenly between its waves);
+// the host shrinks it where the LDS it takes would cost a workgroup per CU
+inline constexpr int kQueueCap = 8192;
+// LDS queue header: one fill counter per wave (at most 16 waves of 64 lanes)
+inline constexpr int kQueueHead = 64;
 // largest global re-bin queue (entries of 16 bytes: 4 GiB); a call needing more is redone with
 // the binary64 kernel
 inline constexpr int64_t kQueueMax = 1LL << 28;
@@ -170,8 +173,8 @@ inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int nce
   L.vE = o; o += 8 * ((n_val + 2) & ~1);
   L.vE32 = o; o += 4 * ((n_val + 4) & ~3);
   L.stage = o; o += stage_bytes;
-  // re-bin queue: a 16-byte counter word, then qcap (i, j) entries of 8 bytes
-  L.q = o; o += qcap > 0 ? 16 + 8 * qcap : 0;
+  // re-bin queue: the per-wave fill counters, then qcap (i, j) entries of 8 bytes
+  L.q = o; o += qcap > 0 ? kQueueHead + 8 * qcap : 0;
   L.total = o;
   return L;
 }

@@ -464,7 +464,8 @@ struct Smem {
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
   bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
   unsigned long long *hq;
-  int qcap;                 // re-bin queue staged in LDS: capacity, fill counter, (i, j) entries
+  int qcap;                 // re-bin queue staged in LDS: entries in all
+  int qw;                   // entries per wave: wave w owns q[w qw, (w + 1) qw) and fill qn[w]
   unsigned *qn;
   int2 *q;
 };
@@ -494,8 +495,9 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.vE32 = (float *)(base + L.vE32);
   m.stage = base + L.stage;
   m.qcap = P.qcap;
+  m.qw = P.qw;
   m.qn = (unsigned *)(base + L.q);
-  m.q = (int2 *)(base + L.q + 16);
+  m.q = (int2 *)(base + L.q + kQueueHead);
   return m;
 }
 
@@ -1047,62 +1049,88 @@ DEV T wave_reserve(T *counter, T total) {
   return __shfl(base, leader);
 }
 
+// the wave's index in its workgroup, in an SGPR
+DEV int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// Orders the wave's LDS accesses to its queue segment: entries written by one lane are read by
+// another lane of the same wave.
+DEV void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Move the first `fill` entries of the wave's LDS segment to the global queue: one reservation
+// for the wave, the entries spread over its active lanes.
+DEV void queue_wave_move(const Params &P, const Smem &m, unsigned fill, int slice_id, int fields) {
+  const unsigned long long act = __ballot(1);
+  const unsigned na = (unsigned)__popcll(act);
+  const unsigned rank = (unsigned)__popcll(act & ((1ull << __lane_id()) - 1ull));
+  const unsigned long long base = wave_reserve<unsigned long long>(P.pending_count, (unsigned long long)fill);
+  wave_lds_sync();
+  const int2 *seg = m.q + wave_index() * m.qw;
+  for (unsigned k = rank; k < fill; k += na) {
+    const unsigned long long idx = base + k;
+    if (idx < (unsigned long long)P.pending_cap) {
+      const int2 e = seg[k];
+      P.pending[idx] = Pending{slice_id, e.x, e.y, fields};
+    }
+  }
+  wave_lds_sync();
+}
+
 // Queue the pairs (i, jg + t) whose bit t is set in dm for the binary64 re-bin, as record
 // indices (the re-bin kernel maps them to the caller's indices).  Called by every active lane of
 // the wave after a group of columns, outside the pair loop: a branch there (with its global atomic
-// and lane exchange) splits the loop body into pieces the compiler cannot overlap.  Entries go to
-// the workgroup's LDS queue, which moves to the global queue once per work item (queue_flush);
-// entries past its capacity go to the global queue directly.  Both reservations are one atomic
-// per wave: the global queue counter is shared by every wave of the device, and under that
-// contention a returning atomic costs thousands of cycles.
+// and lane exchange) splits the loop body into pieces the compiler cannot overlap.  Each wave owns
+// a segment of the workgroup's LDS queue and moves it to the global queue itself when the next
+// group would not fit, and when the slice changes (queue_wave_flush): one global reservation per
+// segment rather than per group.  The global queue counter is shared by every wave of the device,
+// and under that contention a returning atomic costs thousands of cycles; the waves of a
+// workgroup never wait on each other for the queue.  A group larger than a segment goes to the
+// global queue directly.
 DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int jg, int slice_id,
                         int fields = 0) {
   if (__ballot(dm != 0u) == 0ull) return;  // the common case: nothing to queue in the wave
   const unsigned n = (unsigned)__popc(dm);
   unsigned tot;
-  const unsigned pre = wave_prefix(n, &tot);
-  if (tot == 0u) return;  // uniform
-  unsigned k0 = (unsigned)m.qcap;
-  if (m.qcap > 0) k0 = wave_reserve<unsigned>(m.qn, tot) + pre;
-  const unsigned cap = (unsigned)m.qcap;
-  const unsigned n_lds = k0 >= cap ? 0u : (n < cap - k0 ? n : cap - k0);
-  unsigned tot_o;
-  const unsigned pre_o = wave_prefix(n - n_lds, &tot_o);
-  unsigned long long g = 0;
-  if (tot_o) g = wave_reserve<unsigned long long>(P.pending_count, (unsigned long long)tot_o) + pre_o;
-  unsigned e = 0;
+  const unsigned pre = wave_prefix(n, &tot);  // tot > 0: some lane has a bit set
+  const unsigned qw = (unsigned)m.qw;
+  if (tot <= qw) {
+    unsigned *fillp = m.qn + wave_index();
+    unsigned fill = (unsigned)__builtin_amdgcn_readfirstlane((int)*fillp);
+    if (fill + tot > qw) {
+      queue_wave_move(P, m, fill, slice_id, fields);
+      fill = 0u;
+    }
+    int2 *seg = m.q + wave_index() * m.qw + fill + pre;
+    unsigned e = 0;
+    while (dm) {
+      const int t = __ffs((int)dm) - 1;
+      dm &= dm - 1u;
+      seg[e++] = make_int2(i, jg + t);
+    }
+    *fillp = fill + tot;  // every active lane stores the same value
+    return;
+  }
+  unsigned long long g = wave_reserve<unsigned long long>(P.pending_count, (unsigned long long)tot) + pre;
   while (dm) {
     const int t = __ffs((int)dm) - 1;
     dm &= dm - 1u;
-    if (e < n_lds) {
-      m.q[k0 + e] = make_int2(i, jg + t);
-    } else {
-      if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, fields};
-      ++g;
-    }
-    ++e;
+    if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, fields};
+    ++g;
   }
 }
 
-// Move the workgroup's LDS queue to the global queue (all threads; between barriers).
-DEV void queue_flush(const Params &P, const Smem &m, int slice_id, unsigned long long *s_base,
-                     int fields = 0) {
-  if (m.qcap == 0) return;
-  const unsigned n0 = *m.qn;
-  const unsigned n = n0 < (unsigned)m.qcap ? n0 : (unsigned)m.qcap;
-  if (n == 0u) return;  // uniform: every thread read the same counter after the barrier
-  if (threadIdx.x == 0) *s_base = atomicAdd(P.pending_count, (unsigned long long)n);
-  __syncthreads();
-  const unsigned long long base = *s_base;
-  for (unsigned k = threadIdx.x; k < n; k += blockDim.x) {
-    const unsigned long long idx = base + k;
-    if (idx < (unsigned long long)P.pending_cap) {
-      const int2 e = m.q[k];
-      P.pending[idx] = Pending{slice_id, e.x, e.y, fields};
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) *m.qn = 0u;
+// Move the wave's queued entries (pairs of slice slice_id) to the global queue.  Called by whole
+// waves, before the workgroup moves to another slice and before it ends.
+DEV void queue_wave_flush(const Params &P, const Smem &m, int slice_id, int fields = 0) {
+  if (m.qw == 0) return;
+  unsigned *fillp = m.qn + wave_index();
+  const unsigned fill = (unsigned)__builtin_amdgcn_readfirstlane((int)*fillp);
+  if (fill == 0u) return;
+  queue_wave_move(P, m, fill, slice_id, fields);
+  *fillp = 0u;
 }
 
 template <int G, int D, bool J, int OPK, bool MASK, bool FX, bool VU, bool SL>
@@ -1162,13 +1190,12 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
-  __shared__ unsigned long long s_qbase;
   const int B = blockDim.x;
   int cur = -1;
   Slice s{};
   unsigned long long since_flush = 0;
   zero_hist(m);
-  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
+  if ((threadIdx.x & 63) == 0 && m.qcap > 0) m.qn[threadIdx.x >> 6] = 0u;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -1183,6 +1210,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
         __syncthreads();
       }
       if (w.slice != cur) {
+        if (cur >= 0) queue_wave_flush(P, m, cur);
         cur = w.slice;
         s = P.slices[cur];
         load_tables(m, P, s, true);
@@ -1223,11 +1251,12 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
         }
       }
     }
-    __syncthreads();
-    queue_flush(P, m, cur, &s_qbase);
   }
   __syncthreads();
-  if (cur >= 0) flush_hist(m, P, s);
+  if (cur >= 0) {
+    queue_wave_flush(P, m, cur);
+    flush_hist(m, P, s);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1411,13 +1440,12 @@ __global__ void __launch_bounds__(1024) hist32s(Params P, const float *__restric
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
-  __shared__ unsigned long long s_qbase;
   const int B = blockDim.x;
   int cur = -1, nf = 1;
   Slice s{};
   unsigned long long since_flush = 0;
   zero_hist(m);
-  if (threadIdx.x == 0 && m.qcap > 0) *m.qn = 0u;
+  if ((threadIdx.x & 63) == 0 && m.qcap > 0) m.qn[threadIdx.x >> 6] = 0u;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(P.counter, 1);
@@ -1432,6 +1460,7 @@ __global__ void __launch_bounds__(1024) hist32s(Params P, const float *__restric
         __syncthreads();
       }
       if (w.slice != cur) {
+        if (cur >= 0) queue_wave_flush(P, m, cur, nf);
         cur = w.slice;
         s = P.slices[cur];
         nf = s.strip_n > 1 ? s.strip_n : 1;
@@ -1480,11 +1509,12 @@ __global__ void __launch_bounds__(1024) hist32s(Params P, const float *__restric
         else run_chunk_strip<G, D, OPK, false>(P, s, m, ri, rv, nf, i, jb, cnt, cur);
       }
     }
-    __syncthreads();
-    queue_flush(P, m, cur, &s_qbase, nf);
   }
   __syncthreads();
-  if (cur >= 0) flush_strip(m, P, cur, nf);
+  if (cur >= 0) {
+    queue_wave_flush(P, m, cur, nf);
+    flush_strip(m, P, cur, nf);
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
