@@ -710,6 +710,14 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
 // ---------------------------------------------------------------------------------------------
 DEV float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// Nearly coincident points go to the binary64 re-bin: the binary32 records carry positions as
+// hi + lo (2^-48 relative), a direction error of 2^-48 / separation that below a separation of
+// 2^-22 (sphere: radians; flat: times max|x|) would outgrow the value bound's margin and the
+// reference's own binary64 accuracy (2^-53 / separation).  Smallest sin^2(sigma) (sphere) and
+// squared distance (flat) evaluated in binary32:
+inline constexpr float kCloseS2 = 0x1p-44f;
+DEV float close_key(const Slice &s) { return __builtin_fmaxf(1e-30f, 0x1p12f * s.xerr * s.xerr); }
+
 DEV int cell32(float key, const Slice &s) {
   const int bits = __float_as_int(key);
   int c = (bits >> (23 - s.M32)) - s.base32;
@@ -923,7 +931,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
         tn2 = __builtin_fmaf(cz, cz, __builtin_fmaf(cy, cy, cx * cx)) * (rinv * rinv);
       }
     }
-    amb = amb || !(key > 1e-30f);
+    amb = amb || !(key > close_key(s));
   } else {
     // geodesic frame (HelperFunctions.jl:186-200): t_A = (d - (p.d) p)/sin, t_B = (d + (p.d) q)/sin
     // with d = q - p, sin^2(sigma) = |p x d|^2.  U_A is tangent at p and U_B at q, so
@@ -937,7 +945,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
     w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
     const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
-    amb = amb || !(s2 > 1e-30f);
+    amb = amb || !(s2 > kCloseS2);
     const float inv_s = frsq(s2);
     L = __builtin_fmaf(d[2], dU[2], __builtin_fmaf(d[1], dU[1], d[0] * dU[0])) * inv_s;
     float dw = 0.f;
@@ -1330,13 +1338,13 @@ DEV void run_chunk_strip(const Params &P, const Slice &s, const Smem &m, const f
       bool amb = amb_d;
       if constexpr (G == kFlat) {
         rinv = frsq(key);
-        amb = amb || !(key > 1e-30f);
+        amb = amb || !(key > close_key(s));
       } else {
         w[0] = __builtin_fmaf(ri[1], d[2], -ri[2] * d[1]);
         w[1] = __builtin_fmaf(ri[2], d[0], -ri[0] * d[2]);
         w[2] = __builtin_fmaf(ri[0], d[1], -ri[1] * d[0]);
         const float s2 = __builtin_fmaf(w[2], w[2], __builtin_fmaf(w[1], w[1], w[0] * w[0]));
-        amb = amb || !(s2 > 1e-30f);
+        amb = amb || !(s2 > kCloseS2);
         inv_s = frsq(s2);
       }
       float vals[kStripMax];

@@ -124,6 +124,43 @@ def test_parity_all_operators(ctx, geometry, D, joint, prec):
               og=og, mag=MAG[prec])
 
 
+@pytest.mark.parametrize("prec", ["mixed", "fixed"])
+@pytest.mark.parametrize("geometry", ["deg", "flat"])
+def test_nearly_coincident_pairs(ctx, geometry, prec):
+    """Pairs 1e-10 .. 1e-5 apart (sphere: radians; flat: in the unit square), binned down to
+    1e-11 against 800 value bins.  Below a separation of 2^-22 the binary32 records' hi + lo
+    positions (2^-48 relative) would turn the direction by up to 3.5e-5 rad, so those pairs are
+    re-binned in binary64: counts bit-identical and sums within the mixed bar (which a pair
+    evaluated in binary32 at 1e-10 would exceed 30-fold)."""
+    rng = np.random.default_rng(41)
+    m = 1500
+    sep = 10.0 ** rng.uniform(-10, -5, m)
+    ang = rng.uniform(0, 2 * np.pi, m)
+    if geometry == "deg":
+        lon = rng.uniform(-180, 180, m)
+        lat = np.degrees(np.arcsin(rng.uniform(-0.95, 0.95, m)))
+        near = np.stack([lon + np.degrees(sep * np.cos(ang)) / np.cos(np.radians(lat)),
+                         lat + np.degrees(sep * np.sin(ang))])
+        x = np.concatenate([np.stack([lon, lat]), near], axis=1)
+    else:
+        base = rng.uniform(0, 1, (2, m))
+        x = np.concatenate([base, base + sep * np.stack([np.cos(ang), np.sin(ang)])], axis=1)
+    u = rng.normal(size=(2, 2 * m))
+    dedges = np.geomspace(1e-11, 1e-4, 8)
+    ve = np.linspace(-4, 4, 801)
+    os_, oc, oa, og = _oracle.hist(x, u, dedges, op=(0, 1, 0), geometry=geometry, vedges=ve, nthreads=8,
+                                   with_mag=True)
+    gs, gc = gpu_hist(ctx, x, u, dedges, (0, 1, 0), geometry=geometry, vedges=ve, prec=prec)
+    assert oc.sum() >= 0.98 * m, "the close pairs lie in the binned range (a few values lie outside)"
+    assert np.array_equal(gc, oc), f"{geometry} {prec}: counts differ"
+    ok = ~np.isnan(os_)
+    err = np.abs(gs - os_)[ok]
+    bound = TOL["mixed"] * np.maximum(oa, 1e-300) + MAG["mixed"] * og
+    if prec == "fixed":
+        bound = bound + 1.5 * 2.0 ** -23 * (ve[1] - ve[0]) * oc.astype(float)
+    assert np.all(err <= bound[ok]), f"{geometry} {prec}: max excess {np.max(err - bound[ok]):.3g}"
+
+
 @pytest.mark.parametrize("geometry,D", CASES)
 def test_parity_fixed_point_joint(ctx, geometry, D):
     """SF_PREC_MIXED_FIXED: joint cells of uniform value bins as 64-bit fixed-point words.
