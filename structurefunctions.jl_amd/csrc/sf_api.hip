@@ -17,7 +17,6 @@
 #include <algorithm>
 #include <map>
 #include <string>
-#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -41,6 +40,9 @@ hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
 hipError_t launch_derive_sp(double *s, unsigned long long *c, int64_t nbins, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
+hipError_t cull_workspace_bytes(int64_t N, size_t *bytes);
+hipError_t launch_cull(const double *x, int64_t N, int W, int D, int geom, int rows, void *ws,
+                       int32_t *perm, double *info, hipStream_t st);
 int hist_occupancy(int G, int D, bool J, bool mixed, int opk, bool sload, bool fx, int block,
                    size_t smem);
 int hist_opk(const Params &P);
@@ -225,6 +227,9 @@ struct sf_ctx {
   // device buffers: host-input staging and the plan reused by one-shot calls
   DevBuf<double> d_x, d_u;
   Plan oneshot;
+  // spatial sort and tile culling: device workspace and the slice/tile boxes read back
+  DevBuf<char> cull_ws;
+  DevBuf<double> cull_info;
 
   int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -546,6 +551,8 @@ extern "C" void sf_destroy(sf_ctx *ctx) {
   ctx->d_x.release();
   ctx->d_u.release();
   ctx->oneshot.release();
+  ctx->cull_ws.release();
+  ctx->cull_info.release();
   for (int k = 0; k < 4; ++k)
     if (ctx->ev[k]) (void)hipEventDestroy(ctx->ev[k]);
   (void)hipStreamDestroy(ctx->stream);
@@ -627,50 +634,28 @@ namespace {
 // work item (row tile, column chunk) whose bounding boxes are farther apart than the last edge is
 // dropped.  Only the binary32 records are reordered (each keeps its caller index); the binary64
 // records, and with them every re-binned pair, stay in the caller's order.
-// fn(begin, end) over [0, n) split across host threads
-template <typename Fn>
-void parallel_range(int64_t n, Fn fn) {
-  const int nt = (int)std::min<int64_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
-                                        std::max<int64_t>(1, n / 16384));
-  if (nt <= 1) {
-    fn((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t) th.emplace_back(fn, n * t / nt, n * (t + 1) / nt);
-  for (auto &x : th) x.join();
-}
-
 struct Box {
   double lo[3], hi[3];
 };
 
-uint32_t spread10(uint32_t v) {
-  v &= 1023u;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
 struct CullInfo {
   bool active = false;
   double limit = 0;               // a pair of boxes farther apart than this holds no counted pair
-  std::vector<int32_t> perm;      // record k is point perm[k]
-  std::vector<Box> tiles;         // bounding box of each `rows`-point tile, in record order
+  std::vector<Box> tiles;         // bounding box of each `rows`-record tile, in record order
 };
 
+// Spatial order and tile boxes of one slice, computed on the device (launch_cull): the record
+// order goes to perm (device, N entries); ci.active says whether culling is worth it.
 int cull_prepare(sf_ctx *ctx, const sf_problem &p, const double *dx, double xmax, int rows,
-                 hipStream_t st, CullInfo &ci) {
+                 hipStream_t st, CullInfo &ci, int32_t *perm) {
   ci.active = false;
   const int64_t N = p.N;
   const double emax = p.dist_edges[p.n_dist];
   if (N < 4 * (int64_t)rows || !(emax < INFINITY)) return SF_OK;
-  const bool flat0 = p.geometry == SF_GEOM_FLAT;
+  const bool flat = p.geometry == SF_GEOM_FLAT;
   // largest distance in key-space units: flat |dx|; sphere chord 2 sin(theta / 2)
   double reach;
-  if (flat0) {
+  if (flat) {
     reach = std::max(emax, 0.0);
   } else {
     const double R = p.geometry == SF_GEOM_SPHERE_DEG ? p.radius : 1.0;
@@ -679,77 +664,31 @@ int cull_prepare(sf_ctx *ctx, const sf_problem &p, const double *dx, double xmax
   }
   // cheap refusals before touching the data: the sphere's extent is at most 2, a flat slice's
   // at most 2 max|x| (when the caller measured it)
-  if (!flat0 && reach >= 1.0) return SF_OK;
-  if (flat0 && xmax > 0 && reach >= xmax) return SF_OK;
-  std::vector<double> hx((size_t)p.W * N);
-  HIPCHK(hipMemcpyAsync(hx.data(), dx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+  if (!flat && reach >= 1.0) return SF_OK;
+  if (flat && xmax > 0 && reach >= xmax) return SF_OK;
+  const int64_t ntile = (N + rows - 1) / rows;
+  size_t wsb = 0;
+  HIPCHK(cull_workspace_bytes(N, &wsb));
+  HIPCHK(ctx->cull_ws.ensure(wsb));
+  HIPCHK(ctx->cull_info.ensure((size_t)(8 + 6 * ntile)));
+  const int geom = flat ? 0 : (p.geometry == SF_GEOM_SPHERE_DEG ? 1 : 2);
+  HIPCHK(launch_cull(dx, N, p.W, p.D, geom, rows, ctx->cull_ws.p, perm, ctx->cull_info.p, st));
+  std::vector<double> info((size_t)(8 + 6 * ntile));
+  HIPCHK(hipMemcpyAsync(info.data(), ctx->cull_info.p, sizeof(double) * info.size(), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  const bool flat = p.geometry == SF_GEOM_FLAT;
-  std::vector<double> kc((size_t)3 * N, 0.0);
-  parallel_range(N, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; ++i) {
-      if (flat) {
-        for (int k = 0; k < p.D; ++k) kc[3 * i + k] = hx[(size_t)p.W * i + k];
-      } else {
-        const double f = p.geometry == SF_GEOM_SPHERE_DEG ? M_PI / 180.0 : 1.0;
-        const double lon = hx[2 * i] * f, lat = hx[2 * i + 1] * f;
-        kc[3 * i] = cos(lat) * cos(lon);
-        kc[3 * i + 1] = cos(lat) * sin(lon);
-        kc[3 * i + 2] = sin(lat);
-      }
-    }
-  });
-  double cmax = 1.0;
-  for (double v : kc) {
-    if (!(fabs(v) < INFINITY)) return SF_OK;  // non-finite coordinates: no culling
-    cmax = std::max(cmax, fabs(v));
-  }
+  if (info[6] != 0.0) return SF_OK;  // non-finite coordinates: no culling
   // margin: relative 1e-6 plus the rounding of coordinates of magnitude cmax
-  ci.limit = reach * (1.0 + 1e-6) + 1e-13 * cmax;
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = 0; i < N; ++i)
-    for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], kc[3 * i + k]);
-      hi[k] = std::max(hi[k], kc[3 * i + k]);
-    }
+  ci.limit = reach * (1.0 + 1e-6) + 1e-13 * std::max(1.0, info[7]);
   // worth it only if the reach is small against the extent
   double ext = 0;
-  for (int k = 0; k < 3; ++k) ext = std::max(ext, hi[k] - lo[k]);
+  for (int k = 0; k < 3; ++k) ext = std::max(ext, info[3 + k] - info[k]);
   if (!(ci.limit < 0.5 * ext)) return SF_OK;
-  std::vector<uint32_t> key((size_t)N);
-  parallel_range(N, [&](int64_t b, int64_t e) {
-    for (int64_t i = b; i < e; ++i) {
-      uint32_t q[3];
-      for (int k = 0; k < 3; ++k) {
-        const double w = hi[k] - lo[k];
-        const double t = w > 0 ? (kc[3 * i + k] - lo[k]) / w : 0.0;
-        q[k] = (uint32_t)std::min(1023.0, std::max(0.0, floor(t * 1024.0)));
-      }
-      key[i] = spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+  ci.tiles.resize((size_t)ntile);
+  for (int64_t t = 0; t < ntile; ++t)
+    for (int k = 0; k < 3; ++k) {
+      ci.tiles[t].lo[k] = info[8 + 6 * t + k];
+      ci.tiles[t].hi[k] = info[8 + 6 * t + 3 + k];
     }
-  });
-  // stable LSD radix sort of the 30-bit keys, 10 bits per pass
-  ci.perm.resize((size_t)N);
-  std::vector<int32_t> tmp((size_t)N);
-  for (int64_t i = 0; i < N; ++i) ci.perm[i] = (int32_t)i;
-  for (int pass = 0; pass < 3; ++pass) {
-    const int sh = 10 * pass;
-    std::vector<int64_t> cnt(1025, 0);
-    for (int64_t k = 0; k < N; ++k) cnt[((key[ci.perm[k]] >> sh) & 1023u) + 1]++;
-    for (int b = 0; b < 1024; ++b) cnt[b + 1] += cnt[b];
-    for (int64_t k = 0; k < N; ++k) tmp[cnt[(key[ci.perm[k]] >> sh) & 1023u]++] = ci.perm[k];
-    ci.perm.swap(tmp);
-  }
-  const int64_t nt = (N + rows - 1) / rows;
-  ci.tiles.assign((size_t)nt, Box{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}});
-  for (int64_t k = 0; k < N; ++k) {
-    Box &b = ci.tiles[k / rows];
-    const int64_t i = ci.perm[k];
-    for (int c = 0; c < 3; ++c) {
-      b.lo[c] = std::min(b.lo[c], kc[3 * i + c]);
-      b.hi[c] = std::max(b.hi[c], kc[3 * i + c]);
-    }
-  }
   ci.active = true;
   return SF_OK;
 }
@@ -985,26 +924,14 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   if (pl.fx) jchunk = std::max(1, std::min<int>(jchunk, (int)(kFxFlushLimit / block)));
   // spatial sort and tile culling (binary32 kernel, one-shot calls): column chunks of one tile
   std::vector<CullInfo> cinfo(n);
-  std::vector<int32_t> perm_all;
   pl.culled = false;
   if (cull && mixed && !strip) {
+    // the record order of slice s lands at pl.perm + pt_off (used only where culling is active)
+    HIPCHK(pl.perm.ensure((size_t)pl.total_pts));
     for (int s = 0; s < n; ++s) {
-      const int rc = cull_prepare(ctx, sl[s], dx[s], xmax[s], block, st, cinfo[s]);
+      const int rc = cull_prepare(ctx, sl[s], dx[s], xmax[s], block, st, cinfo[s], pl.perm.p + prep[s].pt_off);
       if (rc) return rc;
-    }
-    bool any = false;
-    for (int s = 0; s < n; ++s) any = any || cinfo[s].active;
-    if (any) {
-      perm_all.assign((size_t)pl.total_pts, 0);
-      for (int s = 0; s < n; ++s) {
-        for (int64_t k = 0; k < sl[s].N; ++k)
-          perm_all[prep[s].pt_off + k] = cinfo[s].active ? cinfo[s].perm[k] : (int32_t)k;
-      }
-      HIPCHK(pl.perm.ensure(perm_all.size()));
-      HIPCHK(hipMemcpyAsync(pl.perm.p, perm_all.data(), sizeof(int32_t) * perm_all.size(),
-                            hipMemcpyHostToDevice, st));
-      for (int s = 0; s < n; ++s)
-        if (cinfo[s].active) prep[s].perm = pl.perm.p + prep[s].pt_off;
+      if (cinfo[s].active) prep[s].perm = pl.perm.p + prep[s].pt_off;
     }
   }
   HIPCHK(pl.prep.ensure(n));
