@@ -33,6 +33,17 @@ for p in (PKG, os.path.join(ROOT, "tests")):
 # common spherical part 43 (differences, chord^2, chord^2-space bin, sin^2 sigma, ok test, rsq,
 # tangent projections) + du_L 8 + linear value bin 8 + one accumulation 1.
 FLOPS_PER_PAIR = {"sphere_joint_L1": 60, "flat2_L2": 28}
+# algorithmic FLOP-equivalents per pair of every benchmark configuration's kernels (FMA 2, other
+# arithmetic 1, binary32 transcendental 4; integer and LDS work 0): flat 2D one operator per pass,
+# flat 3D joint du_L, sphere 1D operators and the sphere joint du_L
+TABLE_FLOPS = {("cfg1", "L2"): 28,
+               ("cfg2", "L1"): 27, ("cfg2", "T1"): 30, ("cfg2", "L2"): 28, ("cfg2", "T2"): 32,
+               ("cfg2", "L3"): 29, ("cfg2", "T3"): 32,
+               ("cfg3", "joint L1"): 41,
+               ("cfg4", "L2"): 53, ("cfg4", "T2"): 53, ("cfg4", "L3"): 54, ("cfg4", "T3"): 54,
+               ("cfg4", "joint L1"): 60}
+OPS = {"L1": (0, 1, 0), "T1": (0, 0, 1), "L2": (0, 2, 0), "T2": (0, 0, 2), "L3": (0, 3, 0),
+       "T3": (0, 0, 3), "joint L1": (0, 1, 0)}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 METRIC = "Gpairs/s + % FP32 VALU roofline; mixed-prec vs FP64 GPU path at fixed error"
@@ -68,7 +79,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device copy of 1 GiB (counter calibration for HBM traffic)")
-    ap.add_argument("--workload", choices=["sphere", "batch", "local", "single_pass_2d", "aux"], default="sphere",
+    ap.add_argument("--workload", choices=["sphere", "batch", "local", "single_pass_2d", "aux", "table"],
+                    default="sphere",
                     help="sphere: the headline (N=262144 spherical joint); batch: 256 ragged 2D time "
                          "slices in one plan (hipGraph) against sequential binary64 launches; local: "
                          "the sphere field binned only up to 0.05 rad (spatial sort and tile culling "
@@ -92,6 +104,8 @@ def main():
         return bench_single_pass_2d(args)
     if args.workload == "aux":
         return bench_aux(args)
+    if args.workload == "table":
+        return bench_table(args)
 
     import numpy as np
     import torch
@@ -312,6 +326,78 @@ def bench_single_pass_2d(args):
            "counts_bit_exact": bool(np.array_equal(c32, c64)),
            "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
            "rebinned_pairs": int(st32["rebinned"]), "kernel_ms": round(st32["hist_ns"] * 1e-6, 3)}
+    print(json.dumps(out))
+
+
+def bench_table(args):
+    """Roofline of every benchmark configuration's kernels: for each (configuration, operator) the
+    binary32 kernel's time from the library's HIP events (best of --steps calls after one warm-up)
+    and the whole call's wall time; achieved = pairs x FLOP-equivalents per pair / kernel time,
+    against the FP32 peak recomputed from the device's compute units and clock (256 FLOP per clock
+    per CU) and against the published 157.3 TFLOP/s.  One JSON line."""
+    import numpy as np
+    import torch
+
+    from sfamd import _native, synthetic
+
+    ctx = _native.Context(0)
+    info = ctx.device_info()
+    peak_dev = info["cus"] * 256 * info["clock_khz"] * 1e3 / 1e12
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream().cuda_stream
+    cases = []
+    x1, u1, e1 = synthetic.config1()
+    cases.append(("cfg1", "flat 2D N=4096", x1, u1, e1, None, 0, ["L2"]))
+    x2, u2, e2 = synthetic.config2()
+    cases.append(("cfg2", "flat 2D N=65536", x2, u2, e2, None, 0, ["L1", "T1", "L2", "T2", "L3", "T3"]))
+    x3, u3, e3, v3 = synthetic.config3()
+    cases.append(("cfg3", "flat 3D N=131072, 48x128", x3, u3, e3, v3, 0, ["joint L1"]))
+    x4, u4, e4, v4 = synthetic.config4()
+    cases.append(("cfg4", "sphere N=262144 (degrees, Haversine(1)), 64 distance bins", x4, u4, e4, v4, 1,
+                  ["L2", "T2", "L3", "T3", "joint L1"]))
+    rows = []
+    for cfg, desc, x, u, de, ve, geom, ops in cases:
+        N, D, W = x.shape[1], u.shape[0], x.shape[0]
+        pairs = N * (N - 1) // 2
+        dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+        du = torch.from_numpy(np.ascontiguousarray(u.T)).to(dev)
+        for name in ops:
+            joint = name.startswith("joint")
+            vedges = ve if joint else None
+            ncell = (len(de) - 1) * ((len(ve) - 1) if joint else 1)
+            s = torch.zeros(ncell, dtype=torch.float64, device=dev)
+            c = torch.zeros(ncell, dtype=torch.int64, device=dev)
+            precs = ["mixed_fixed", "mixed"] if joint else ["mixed"]
+            for prec in precs:
+                p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, D, W, geom, OPS[name], de,
+                                    vedges, prec)
+                best_k = best_w = None
+                for r in range(max(1, args.steps) + 1):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    ctx.hist_device([p], s.data_ptr(), c.data_ptr(), stream)
+                    torch.cuda.synchronize()
+                    wall = time.perf_counter() - t0
+                    kern = ctx.stats()["hist_ns"] * 1e-9
+                    if r > 0:
+                        best_k = kern if best_k is None else min(best_k, kern)
+                        best_w = wall if best_w is None else min(best_w, wall)
+                f = TABLE_FLOPS[(cfg, name)]
+                ach = pairs * f / best_k / 1e12
+                rows.append({"config": cfg, "workload": desc, "operator": name, "precision": prec,
+                             "pairs": pairs, "flops_per_pair": f,
+                             "kernel_ms": round(best_k * 1e3, 3), "call_ms": round(best_w * 1e3, 3),
+                             "kernel_gpairs_s": round(pairs / best_k / 1e9, 2),
+                             "call_gpairs_s": round(pairs / best_w / 1e9, 2),
+                             "achieved_tflops": round(ach, 2),
+                             "frac_published": round(ach / PEAK_FP32_TFLOPS, 4),
+                             "frac_device": round(ach / peak_dev, 4)})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        del dx, du
+    out = {"metric": "per-configuration roofline of the binary32 kernels (kernel-time FLOP-equivalents)",
+           "unit": "TFLOP/s", "peak_published": PEAK_FP32_TFLOPS, "peak_device": round(peak_dev, 2),
+           "device": info, "steps": args.steps, "data": "synthetic (seeded configurations 1-4)",
+           "rows": rows}
     print(json.dumps(out))
 
 

@@ -183,6 +183,11 @@ void sf_plan_destroy(sf_plan *plan);
  * around the launches; reading them waits for that stream.  Returns 0. */
 int sf_last_stats(sf_ctx *ctx, int64_t out[8]);
 
+/* The context's device, for roofline figures: out[0] = compute units, out[1] = peak shader clock
+ * (kHz), out[2] = LDS bytes a workgroup may use, out[3] = lanes per wavefront.  No counterpart in
+ * the reference (its GPU extensions report no device figures).  Returns 0. */
+int sf_device_info(sf_ctx *ctx, int64_t out[4]);
+
 /* Validation only, no device work and no context needed: returns the code sf_hist_batch would
  * return for these slices before launching, and writes the message into msg (if not NULL).
  * Mirrors the reference's pre-loop checks (shapes.jl:67-105, serial.jl:111-120,

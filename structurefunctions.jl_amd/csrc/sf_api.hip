@@ -214,6 +214,7 @@ struct sf_ctx {
   hipStream_t stream = nullptr;
   int n_cu = 0;
   size_t max_smem = 0;
+  int clock_khz = 0;   // peak shader clock
   std::string err;
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -536,6 +537,7 @@ extern "C" int sf_create(int device, sf_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return SF_E_DEVICE; }
   ctx->n_cu = prop.multiProcessorCount;
   ctx->max_smem = prop.sharedMemPerBlock;
+  ctx->clock_khz = prop.clockRate;
   int v = 0;
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
     ctx->max_smem = (size_t)v;
@@ -560,6 +562,15 @@ extern "C" void sf_destroy(sf_ctx *ctx) {
 }
 
 extern "C" const char *sf_last_error(const sf_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int sf_device_info(sf_ctx *ctx, int64_t out[4]) {
+  if (!ctx || !out) return SF_E_ARG;
+  out[0] = ctx->n_cu;
+  out[1] = ctx->clock_khz;
+  out[2] = (int64_t)ctx->max_smem;
+  out[3] = 64;
+  return SF_OK;
+}
 
 extern "C" int sf_last_stats(sf_ctx *ctx, int64_t out[8]) {
   if (!ctx || !out) return SF_E_ARG;

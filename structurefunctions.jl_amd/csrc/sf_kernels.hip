@@ -785,19 +785,26 @@ DEV float op_value32(int op, int nl, int nt, int D, float L, float T, float tn2,
 }
 
 // Operator class fixed at compile time: 0 = any operator (runtime switch), 1 = du_L
-// (ProjectedStructureFunctionType{1,0}), 2 = du_L^2 ({2,0}).
+// (ProjectedStructureFunctionType{1,0}), 2 = du_L^2 ({2,0}); binary32 1D kernels also 5 = du_L^3
+// ({3,0}), 6 = du_T ({0,1}), 7 = du_T^2 ({0,2}), 8 = du_T^3 ({0,3}), with the operation order of
+// the runtime switch (op_value32).
 template <int OPK>
 DEV float op32(const Params &P, int D, float L, float T, float tn2, float n2) {
   if constexpr (OPK == 1) return L;
   else if constexpr (OPK == 2) return L * L;
+  else if constexpr (OPK == 5) return L * L * L;
+  else if constexpr (OPK == 6) return T;
+  else if constexpr (OPK == 7) return tn2;
+  else if constexpr (OPK == 8) return T * T * T;
   else return op_value32(P.op, P.nl, P.nt, D, L, T, tn2, n2);
 }
 
 // (|u_i| + |u_j|)^order without a loop (order 0..3)
 template <int OPK>
 DEV float pow_order(const Params &P, float x) {
-  if constexpr (OPK == 1) return x;
-  else if constexpr (OPK == 2) return x * x;
+  if constexpr (OPK == 1 || OPK == 6) return x;
+  else if constexpr (OPK == 2 || OPK == 7) return x * x;
+  else if constexpr (OPK == 5 || OPK == 8) return x * x * x;
   else {
     const float x2 = x * x;
     return P.order == 0 ? 1.0f : (P.order == 1 ? x : (P.order == 2 ? x2 : x2 * x));
@@ -836,7 +843,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   // |du|^2 = du_L^2 + |du_T|^2
   constexpr bool SP = (OPK == 3);
   constexpr bool SP2 = (OPK == 4);
-  constexpr bool needT = (OPK == 0) || ((SP || SP2) && G == kSphere);
+  constexpr bool needT = (OPK == 0) || (OPK >= 6) || ((SP || SP2) && G == kSphere);
   constexpr bool needN2 = (OPK == 0) || SP || SP2;
   const int nb = P.n_dist;
   const bool upper = !MASK || j > i;
@@ -1713,6 +1720,12 @@ template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
+    if constexpr (!SL) {  // 1D, LDS-staged columns (the 1D default): L^3, T, T^2, T^3
+      if (opk == 5) return (const void *)&hist32<G, D, false, 5, false, false>;
+      if (opk == 6) return (const void *)&hist32<G, D, false, 6, false, false>;
+      if (opk == 7) return (const void *)&hist32<G, D, false, 7, false, false>;
+      if (opk == 8) return (const void *)&hist32<G, D, false, 8, false, false>;
+    }
   } else {
     if (opk == 4) return (const void *)&hist32<G, D, true, 4, SL, FX>;
   }
@@ -1757,6 +1770,10 @@ static int opk_of(const Params &P) {
   if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
   if (P.op == 0 && P.nl == 2 && P.nt == 0) return 2;
+  if (P.op == 0 && P.nl == 3 && P.nt == 0) return 5;
+  if (P.op == 0 && P.nl == 0 && P.nt == 1) return 6;
+  if (P.op == 0 && P.nl == 0 && P.nt == 2) return 7;
+  if (P.op == 0 && P.nl == 0 && P.nt == 3) return 8;
   return 0;
 }
 

@@ -22,7 +22,8 @@ SEM_POINT, SEM_BATCH = 0, 1
 OP_SINGLE_PASS = 6
 
 EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batch",
-            "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_check", "sf_eval_math",
+            "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_device_info", "sf_check",
+            "sf_eval_math",
             "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy",
             "sf_autotune")
 
@@ -91,6 +92,8 @@ def lib():
     L.sf_hist_batch_device.restype = ctypes.c_int
     L.sf_last_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.sf_last_stats.restype = ctypes.c_int
+    L.sf_device_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.sf_device_info.restype = ctypes.c_int
     L.sf_check.argtypes = [i32, ctypes.POINTER(SfProblem), ctypes.c_char_p, i32]
     L.sf_check.restype = ctypes.c_int
     L.sf_eval_math.argtypes = [vp, i32, i64, dp, dp, dp]
@@ -155,6 +158,12 @@ class Context:
     def check(self, rc: int):
         if rc != SF_OK:
             raise_for(rc, self.error())
+
+    def device_info(self):
+        """Compute units, peak shader clock (kHz), LDS bytes per workgroup, lanes per wavefront."""
+        out = (ctypes.c_int64 * 4)()
+        self.check(lib().sf_device_info(self._h, out))
+        return {"cus": out[0], "clock_khz": out[1], "lds_bytes": out[2], "wave": out[3]}
 
     def stats(self):
         out = (ctypes.c_int64 * 8)()
