@@ -203,7 +203,8 @@ int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a, const doub
 /* Kernel tuning override (autotuner hook): threads per workgroup (= rows of an i-tile; a multiple
  * of 64 in 64..1024),
  * column-range length per work item, workgroups per CU, and the column-record variant of the
- * binary32 kernel (1: staged through LDS, 2: scalar loads).  Zero keeps the built-in choice.
+ * binary32 kernel (1: staged through LDS, 2: scalar loads, 3: B-strips -- 1D slices of one batch
+ * that share positions run four fields per pair evaluation).  Zero keeps the built-in choice.
  * Applies to subsequent calls on this context.  (No reference counterpart; the reference's GPU
  * path takes workgroup_size as a keyword, KA ext :385-401.) */
 int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu,

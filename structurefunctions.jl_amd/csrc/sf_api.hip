@@ -628,7 +628,7 @@ extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_
   if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
     return ctx->fail(SF_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
   if (j_chunk < 0 || blocks_per_cu < 0) return ctx->fail(SF_E_ARG, "tuning values must be >= 0");
-  if (variant < 0 || variant > 2) return ctx->fail(SF_E_ARG, "variant must be 0, 1 or 2");
+  if (variant < 0 || variant > 3) return ctx->fail(SF_E_ARG, "variant must be 0, 1, 2 or 3");
   ctx->tune_variant = variant;
   ctx->tune_block = block;
   ctx->tune_jchunk = j_chunk;
@@ -779,9 +779,12 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   pl.ncell = out_cells(p0);  // per input slice
   pl.sp = p0.op == SF_OP_SINGLE_PASS && !planes;
   // B-strip: slices that share positions (the auxiliary axes of one field: same x, N, edges) and
-  // bin 1D histograms in binary32 evaluate each pair's geometry once for up to kStripMax fields
+  // bin 1D histograms in binary32 evaluate each pair's geometry once for up to kStripMax fields.
+  // Opt-in (tuning variant 3): the per-slice 1D kernel with 16 histogram copies outruns it (eight
+  // sphere fields, N = 65536: 828 against 609 Gpairs/s), its column velocities being LDS reads.
   pl.strip = 0;
-  if (p0.precision != SF_PREC_FP64 && p0.n_val == 0 && p0.op != SF_OP_SINGLE_PASS && n_in >= 2) {
+  if (ctx->tune_variant == 3 && p0.precision != SF_PREC_FP64 && p0.n_val == 0 &&
+      p0.op != SF_OP_SINGLE_PASS && n_in >= 2) {
     bool same = true;
     for (int s = 1; s < n && same; ++s)
       same = sl[s].x == p0.x && sl[s].N == p0.N && xmax[s] == xmax[0] &&
@@ -886,7 +889,10 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   const int stage_bytes = sload ? 0 : tj * rec_bytes * (strip ? strip : 1);
   int copies = 1;
   if (!J) {
-    copies = 64;
+    // per-lane-group copies of a 1D histogram (lanes l and l + copies share one): 16 measured
+    // best on both binary32 1D kernels (flat 2D du_L^2 at N = 65536: 636 / 643 / 840 / 708 / 597
+    // Gpairs/s for 64 / 32 / 16 / 8 / 4 copies; sphere at N = 262144: 601 / 605 / 890 / 732 / 610)
+    copies = 16;
     while (copies > 1 &&
            (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0, strip).total * 2 > ctx->max_smem)
       copies >>= 1;

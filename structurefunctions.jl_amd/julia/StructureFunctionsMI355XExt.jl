@@ -196,9 +196,10 @@ end
 # ---- auxiliary axes: u (D, N, B...) over shared x (W, N) or per-slice x (W, N, B...) -----------
 # The reference runs these through its batch-leading kernels (batch.jl:201-217,
 # batch_leading.jl:105-330), whose flat pairs bin r = sqrt(dx.dx) against the edges: the library
-# follows them with semantics = SF_SEM_BATCH.  Slices share x and the edges, so the library's
-# B-strips evaluate each pair's geometry once for several fields.  Outputs are (n_bins[, n_val],
-# aux...), column-major, like the reference's batch results.
+# follows them with semantics = SF_SEM_BATCH.  Slices share x and the edges; the library runs one
+# pair pass per field (its B-strips, which evaluate each pair's geometry once for four fields,
+# are an opt-in tuning variant).  Outputs are (n_bins[, n_val], aux...), column-major, like the
+# reference's batch results.
 function _run_aux!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
                    dbins, vbins, metric, CT)
     ctx = _context(b.device)

@@ -1,7 +1,8 @@
 """B-strips: slices that share positions (a field with auxiliary axes) and bin 1D histograms are
 run by a kernel that evaluates each pair's geometry once for up to four velocity fields
-(batch_leading.jl:105-140).  Each field must still equal the oracle's batch-leading evaluation
-of its own slice: counts bit-identical, sums within the mixed bound."""
+(batch_leading.jl:105-140), opt-in through tuning variant 3.  Each field must still equal the
+oracle's batch-leading evaluation of its own slice: counts bit-identical, sums within the mixed
+bound."""
 import numpy as np
 import pytest
 
@@ -12,6 +13,16 @@ from sfamd import _native
 pytestmark = pytest.mark.gpu
 
 OPS = [(sfamd.L1SF, (0, 1, 0)), (sfamd.L2SF, (0, 2, 0)), (sfamd.S2SF, (1, 0, 0)), (sfamd.T2SF, (0, 0, 2))]
+
+
+@pytest.fixture(autouse=True)
+def strips_on():
+    """B-strips are opt-in (tuning variant 3): the public API's context runs them for every test
+    here, and goes back to the default afterwards."""
+    ctx = sfamd.MI355XBackend().context()
+    ctx.set_tuning(0, 0, 0, 3)
+    yield
+    ctx.set_tuning(0, 0, 0, 0)
 
 
 def check_fields(r, x, u, e, op, geometry, prec, label):
@@ -69,6 +80,7 @@ def test_strip_runs_one_item_set_per_strip():
     u = rng.normal(size=(2, N, 8))
     e = np.geomspace(1e-2, 1.4, 17)
     ctx = _native.Context(0)
+    ctx.set_tuning(0, 0, 0, 3)
     keep = []
 
     def problems(shared):
@@ -117,6 +129,7 @@ def test_strip_plan_binary64_redo():
     dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
     dus = [torch.from_numpy(np.ascontiguousarray(u[:, :, b].T)).to(dev) for b in range(2)]
     ctx = _native.Context(0)
+    ctx.set_tuning(0, 0, 0, 3)
     probs = []
     for b in range(2):
         p = _native.SfProblem()
