@@ -1101,7 +1101,9 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
-  if (use_mixed) HIPCHK(launch_recheck(pl.G, pl.D, pl.J, P, ctx->n_cu, st));
+  // four 256-thread workgroups per CU: the re-bin is a latency-bound gather (2D single pass,
+  // 2.7e8 queued pairs: 22.6 / 16.7 / 19.6 ms with one / four / eight per CU)
+  if (use_mixed) HIPCHK(launch_recheck(pl.G, pl.D, pl.J, P, 4 * ctx->n_cu, st));
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
   HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
                               use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
