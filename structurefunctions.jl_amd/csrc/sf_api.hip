@@ -469,6 +469,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   s.v_negzero = 0;
   s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0;
   s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0;
+  s.v_log = 0; s.vk0 = 0; s.vl0f = 0; s.vlinvf = 0;
   if (p.n_val > 0) {
     const int nv = p.n_val;
     const double *E = p.val_edges;
@@ -497,6 +498,25 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
       }
     }
     if (!s.v_uniform) s.vslack = (float)ldexpl(1.0L, -21);  // relative slack for binary32 edges
+    if (!s.v_uniform) {
+      // log-spaced from the first positive edge on (a leading 0 or negative edges allowed): the
+      // binary32 kernel guesses the bin from log2(v) and walks to it; the walk alone decides, so
+      // the guess only needs to be close
+      int k0 = 0;
+      while (k0 <= nv && !(E[k0] > 0.0)) ++k0;
+      if (k0 + 3 <= nv && E[nv] < INFINITY) {
+        const long double l0 = log2l((long double)E[k0]), l1 = log2l((long double)E[nv]);
+        const long double st = (l1 - l0) / (nv - k0);
+        long double dev = 0;
+        for (int k = k0; k <= nv; ++k) dev = std::max(dev, fabsl((log2l((long double)E[k]) - l0) / st - (k - k0)));
+        if (st > 0 && dev < 0.25L) {
+          s.v_log = 1;
+          s.vk0 = k0;
+          s.vl0f = (float)l0;
+          s.vlinvf = (float)(1.0L / st);
+        }
+      }
+    }
   }
 }
 

@@ -68,6 +68,10 @@ struct Slice {
   int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
   float xerr;         // flat: 2^-28 max|x| (binary32 value bound: reference's direction error)
   int32_t strip_n;    // B-strip launch: fields of the strip this slice leads (0: not a leader)
+  // value edges log-spaced from edge vk0 on (binary32 lookup guesses the bin from log2 v):
+  // log2(edge vk0 + m) ~ vl0f + m / vlinvf
+  int32_t v_log, vk0;
+  float vl0f, vlinvf;
 };
 
 struct Item {

@@ -746,10 +746,21 @@ DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, b
     gf = gh;
     return (int)gh;
   }
-  int lo = 0, hi = nE;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (E32[mid] < v) lo = mid + 1; else hi = mid;
+  int lo = 0;
+  if (s.v_log) {
+    // log-spaced edges (positive from edge vk0 on): the first edge >= v guessed from log2(v),
+    // then a walk to it, which alone decides (v <= 0 and NaN start from edge 0)
+    float tg = __builtin_fmaf(__builtin_amdgcn_logf(v) - s.vl0f, s.vlinvf, (float)s.vk0 + 1.0f);
+    tg = __builtin_fminf(__builtin_fmaxf(tg, 0.0f), (float)nE);
+    lo = (int)tg;
+    while (lo > 0 && !(E32[lo - 1] < v)) --lo;
+    while (lo < nE && E32[lo] < v) ++lo;
+  } else {
+    int hi = nE;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (E32[mid] < v) lo = mid + 1; else hi = mid;
+    }
   }
   // nearest edges must be farther than the error bound (edges rounded to binary32: slack)
   const float gap = dv + s.vslack * __builtin_fabsf(v) + 1e-37f;
