@@ -982,6 +982,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     const int64_t N = sl[s].N;
     const CullInfo &ci = cinfo[s];
     const int64_t jc = ci.active ? block : jchunk;
+    const size_t first = items.size();
     for (int64_t i0 = 0; i0 < N - 1; i0 += block) {
       for (int64_t j0 = i0; j0 < N; j0 += jc) {
         if (ci.active && j0 > i0 && boxes_apart(ci.tiles[i0 / block], ci.tiles[j0 / block], ci.limit)) {
@@ -998,6 +999,15 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
         if (strip) items64.push_back(it);
       }
     }
+    // largest items of the slice first (rows x columns; a diagonal item costs its full rectangle):
+    // the short last column range of every row tile and the last row tiles end the queue, so
+    // workgroups finish together
+    const int64_t rows = block;
+    std::stable_sort(items.begin() + first, items.end(), [N, rows](const Item &a, const Item &b) {
+      const int64_t ca = std::min<int64_t>(N - a.i0, rows) * (a.j1 - a.j0);
+      const int64_t cb = std::min<int64_t>(N - b.i0, rows) * (b.j1 - b.j0);
+      return ca > cb;
+    });
   }
   if (items.empty()) items.push_back(Item{0, 0, 0, 0});
   if (strip && items64.empty()) items64.push_back(Item{0, 0, 0, 0});
