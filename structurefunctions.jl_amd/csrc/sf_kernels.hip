@@ -1731,11 +1731,11 @@ template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
-    if constexpr (!SL) {  // 1D, LDS-staged columns (the 1D default): L^3, T, T^2, T^3
-      if (opk == 5) return (const void *)&hist32<G, D, false, 5, false, false>;
-      if (opk == 6) return (const void *)&hist32<G, D, false, 6, false, false>;
-      if (opk == 7) return (const void *)&hist32<G, D, false, 7, false, false>;
-      if (opk == 8) return (const void *)&hist32<G, D, false, 8, false, false>;
+    if constexpr (SL) {  // 1D, scalar-loaded columns (the default): L^3, T, T^2, T^3
+      if (opk == 5) return (const void *)&hist32<G, D, false, 5, true, false>;
+      if (opk == 6) return (const void *)&hist32<G, D, false, 6, true, false>;
+      if (opk == 7) return (const void *)&hist32<G, D, false, 7, true, false>;
+      if (opk == 8) return (const void *)&hist32<G, D, false, 8, true, false>;
     }
   } else {
     if (opk == 4) return (const void *)&hist32<G, D, true, 4, SL, FX>;
