@@ -904,7 +904,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   // 204 Gpairs/s at N = 65536 (they also keep the record out of VGPRs, which the two-workgroup
   // fixed-point launch needs); 1D histograms with 16 copies, N = 262144, sphere 974 vs 908 and
   // flat 2D 1158 vs 958 (no LDS stage: more workgroups per CU)
-  const bool sload = mixed && !strip && (t_variant == 2 || t_variant == 0);
+  const bool sload = mixed && !strip && t_variant != 1;
   const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
   // a B-strip stages every field's column records
   const int stage_bytes = sload ? 0 : tj * rec_bytes * (strip ? strip : 1);
