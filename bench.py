@@ -41,9 +41,12 @@ TABLE_FLOPS = {("cfg1", "L2"): 28,
                ("cfg2", "L3"): 29, ("cfg2", "T3"): 32,
                ("cfg3", "joint L1"): 41,
                ("cfg4", "L2"): 53, ("cfg4", "T2"): 53, ("cfg4", "L3"): 54, ("cfg4", "T3"): 54,
-               ("cfg4", "joint L1"): 60}
+               ("cfg4", "joint L1"): 60,
+               # the six single-pass invariants from one pair evaluation: flat 2D 40; sphere: the
+               # common spherical part 43, du_L 8, du_T 8, |du|^2 and the four moments 10
+               ("cfg2", "single pass"): 40, ("cfg4", "single pass"): 69}
 OPS = {"L1": (0, 1, 0), "T1": (0, 0, 1), "L2": (0, 2, 0), "T2": (0, 0, 2), "L3": (0, 3, 0),
-       "T3": (0, 0, 3), "joint L1": (0, 1, 0)}
+       "T3": (0, 0, 3), "joint L1": (0, 1, 0), "single pass": (6, 0, 0)}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 METRIC = "Gpairs/s + % FP32 VALU roofline; mixed-prec vs FP64 GPU path at fixed error"
@@ -349,12 +352,13 @@ def bench_table(args):
     x1, u1, e1 = synthetic.config1()
     cases.append(("cfg1", "flat 2D N=4096", x1, u1, e1, None, 0, ["L2"]))
     x2, u2, e2 = synthetic.config2()
-    cases.append(("cfg2", "flat 2D N=65536", x2, u2, e2, None, 0, ["L1", "T1", "L2", "T2", "L3", "T3"]))
+    cases.append(("cfg2", "flat 2D N=65536", x2, u2, e2, None, 0,
+                  ["L1", "T1", "L2", "T2", "L3", "T3", "single pass"]))
     x3, u3, e3, v3 = synthetic.config3()
     cases.append(("cfg3", "flat 3D N=131072, 48x128", x3, u3, e3, v3, 0, ["joint L1"]))
     x4, u4, e4, v4 = synthetic.config4()
     cases.append(("cfg4", "sphere N=262144 (degrees, Haversine(1)), 64 distance bins", x4, u4, e4, v4, 1,
-                  ["L2", "T2", "L3", "T3", "joint L1"]))
+                  ["L2", "T2", "L3", "T3", "joint L1", "single pass"]))
     rows = []
     for cfg, desc, x, u, de, ve, geom, ops in cases:
         N, D, W = x.shape[1], u.shape[0], x.shape[0]
@@ -364,7 +368,7 @@ def bench_table(args):
         for name in ops:
             joint = name.startswith("joint")
             vedges = ve if joint else None
-            ncell = (len(de) - 1) * ((len(ve) - 1) if joint else 1)
+            ncell = (len(de) - 1) * ((len(ve) - 1) if joint else (6 if name == "single pass" else 1))
             s = torch.zeros(ncell, dtype=torch.float64, device=dev)
             c = torch.zeros(ncell, dtype=torch.int64, device=dev)
             precs = ["mixed_fixed", "mixed"] if joint else ["mixed"]
