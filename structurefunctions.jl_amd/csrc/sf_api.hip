@@ -1000,14 +1000,12 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
         if (strip) items64.push_back(it);
       }
     }
-    // largest items of the slice first (rows x columns; a diagonal item costs its full rectangle):
-    // the short last column range of every row tile and the last row tiles end the queue, so
-    // workgroups finish together
+    // full items of the slice first (a diagonal item costs its full rectangle): the short last
+    // column range of every row tile and the short last row tile end the queue, so workgroups
+    // finish together (a linear-time partition; a one-shot call builds its items every time)
     const int64_t rows = block;
-    std::stable_sort(items.begin() + first, items.end(), [N, rows](const Item &a, const Item &b) {
-      const int64_t ca = std::min<int64_t>(N - a.i0, rows) * (a.j1 - a.j0);
-      const int64_t cb = std::min<int64_t>(N - b.i0, rows) * (b.j1 - b.j0);
-      return ca > cb;
+    std::stable_partition(items.begin() + first, items.end(), [N, rows, jc](const Item &a) {
+      return N - a.i0 >= rows && a.j1 - a.j0 == jc;
     });
   }
   if (items.empty()) items.push_back(Item{0, 0, 0, 0});
