@@ -896,7 +896,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     if (it != ctx->tuned.end()) std::tie(t_block, t_jchunk, t_bpc, t_variant) = it->second;
   }
   // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
-  int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : 256);
+  // 1D single pass: 512 threads (its 36-byte slots fill the LDS at two or three workgroups per CU;
+  // more waves per workgroup then pay: flat 2D 595 against 492 Gpairs/s, sphere 644 against 529)
+  int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : (pl.sp ? 512 : 256));
   // column records staged per LDS chunk; the fixed-point joint launch keeps two workgroups per CU
   const int tj = std::min(block, (pl.fx || strip) ? 128 : 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
