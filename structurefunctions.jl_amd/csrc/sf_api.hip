@@ -897,7 +897,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   }
   // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
   // 1D single pass: 512 threads (its 36-byte slots fill the LDS at two or three workgroups per CU;
-  // more waves per workgroup then pay: flat 2D 595 against 492 Gpairs/s, sphere 644 against 529)
+  // more waves per workgroup then pay: flat 2D 595 against 492 Gpairs/s, sphere 644 against 529).
+  // One-operator 1D kernels keep 256: 512 gained 5 % on flat 2D at N = 262144, nothing at 65536,
+  // and lost on small slices (N = 4096: 51 against 88; the 256 ragged slices: 1019 against 1031)
   int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : (pl.sp ? 512 : 256));
   // column records staged per LDS chunk; the fixed-point joint launch keeps two workgroups per CU
   const int tj = std::min(block, (pl.fx || strip) ? 128 : 256);
