@@ -187,18 +187,27 @@ def main():
         g64 = rate("fp64")
         extra["fp64_gpairs_s"] = round(g64, 3)
         extra["speedup_vs_fp64"] = round(gps / g64, 3)
-        # du_L is odd: bins of a joint (distance, du_L) histogram hold values of one sign, so the
-        # per-bin relative error |s32 - s64| / |s64| is well conditioned
-        big = c64 >= 1000
+        # per-cell relative error |s32 - s64| / |s64| over every non-empty cell.  A joint
+        # (distance, du_L) cell holds values of one sign (the value edge at 0 splits the signs),
+        # so |sum v| = sum |v| there and the ratio is the error relative to the cell's magnitude.
+        # The binary64 kernel's counts equal the oracle's and its sums agree to 1e-12 (the
+        # scale parity tests), so it stands in for the oracle at this size.
+        nz = c64 > 0
         precs = [args.precision] + (["mixed"] if args.precision == "mixed_fixed" else [])
         for prec in precs:
             s32, c32 = result(prec)
-            rel = np.abs(s32 - s64)[big] / np.maximum(np.abs(s64[big]), 1e-300)
+            rel = np.abs(s32 - s64)[nz] / np.maximum(np.abs(s64[nz]), 1e-300)
+            cnt = c64[nz]
+            bad = rel > 1e-6
             extra[f"{prec}_vs_fp64"] = {
                 "counts_bit_exact": bool(np.array_equal(c32, c64)),
-                "sums_max_rel_err": float(rel.max()) if rel.size else None,
-                "sums_median_rel_err": float(np.median(rel)) if rel.size else None,
-                "bins_compared": int(big.sum())}
+                "cells_nonempty": int(nz.sum()),
+                "sums_max_rel_err_all_cells": float(rel.max()) if rel.size else None,
+                "sums_median_rel_err_all_cells": float(np.median(rel)) if rel.size else None,
+                "cells_over_1e-6": int(bad.sum()),
+                "pairs_in_cells_over_1e-6": int(cnt[bad].sum()),
+                "largest_count_of_a_cell_over_1e-6": int(cnt[bad].max()) if bad.any() else 0,
+                "sums_max_rel_err_cells_ge_1000_pairs": float(rel[cnt >= 1000].max()) if (cnt >= 1000).any() else None}
         if args.precision == "mixed_fixed":
             extra["mixed_gpairs_s"] = round(rate("mixed"), 3)
 
@@ -238,9 +247,9 @@ def main():
         "data": "synthetic (seeded Fourier-mode field on the sphere)",
         "config": {"workload": f"sphere N={N} Haversine(1) joint du_L x distance 64x96, {args.precision}",
                    "pairs_per_step": pairs, "precision": args.precision, "rebinned_pairs": int(rebinned)},
-        # compute-bound: binary32 vector arithmetic (no matrix instructions); 157.3 TFLOP/s is the
-        # published fp32 peak, the same figure for the vector and matrix pipes
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+        # compute-bound on the vector ALU (binary32 arithmetic, no matrix instructions); 157.3
+        # TFLOP/s is the published fp32 vector peak
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": read_traffic(TRAFFIC_FILE), "pipe": "VALU fp32",
                      "kernel": "hist32<sphere,D=2,joint,du_L" + (",fixed-point cells>" if args.precision == "mixed_fixed" else ">")
@@ -535,7 +544,8 @@ def bench_batch(args):
         step_batched()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    rate = 2 * pairs / dt / 1e9
+    # Gpairs/s: N(N-1)/2 summed over the slices per wall second (each pair feeds both outputs)
+    rate = pairs / dt / 1e9
 
     # sequential binary64 launches, one per slice and operator
     seq_s = torch.zeros(nd * nv, dtype=torch.float64, device=dev)
@@ -554,7 +564,7 @@ def bench_batch(args):
     t0 = time.perf_counter()
     step_sequential()
     dt_seq = time.perf_counter() - t0
-    rate_seq = 2 * pairs / dt_seq / 1e9
+    rate_seq = pairs / dt_seq / 1e9
 
     # binary64 in the same batched plan form
     p64_1 = _native.Plan(ctx, probs((0, 2, 0), False, "fp64"))
@@ -571,10 +581,11 @@ def bench_batch(args):
     t0 = time.perf_counter()
     step64()
     torch.cuda.synchronize()
-    rate_b64 = 2 * pairs / (time.perf_counter() - t0) / 1e9
+    rate_b64 = pairs / (time.perf_counter() - t0) / 1e9
 
-    out = {"metric": "Gpairs/s, batched ragged time slices (two operators per slice)",
-           "value": round(rate, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
+    out = {"metric": "Gpairs/s, batched ragged time slices (N(N-1)/2 summed over slices per second; "
+                     "two outputs per slice)",
+           "value": round(rate, 3), "pair_evaluations_per_s_G": round(2 * rate, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
            "dtype": DTYPE[prec], "data": "synthetic (256 seeded 2D Fourier-mode fields)",
            "config": {"workload": "256 slices, N log-uniform [2000, 40000], L2 1D 48 bins + joint du_L 48x64",

@@ -149,13 +149,16 @@ int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d
 int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
                          uint64_t *d_counts, void *stream);
 
-/* Autotuner: times candidate launch shapes (threads per workgroup, column-range length, column
- * variant) on these slices (device pointers, as for sf_hist_batch_device; outputs are discarded),
- * keeps the fastest in the context keyed by (geometry, D, joint, precision, operator class,
- * log2 of the pair count, n_dist, n_val), and writes it to out (block, j_chunk, blocks_per_cu,
- * variant).  Later calls with a matching key and no sf_set_tuning override use it.  (No reference
- * counterpart.) */
-int sf_autotune(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t out[4]);
+/* Autotuner: times candidate launch shapes on these slices (device pointers, as for
+ * sf_hist_batch_device; outputs are discarded) -- threads per workgroup, column-record variant,
+ * the histogram's privatisation (1D: 4..32 lane-group copies; joint: 1 or 2 copies by lane halves)
+ * and joint slot layout, then the column-range length -- keeps the fastest in the context keyed by
+ * the call's shape (geometry, D, joint, precision, operator class and order, 2D single-pass
+ * planes, log2 of the pair count, n_dist, n_val), and writes it to out (block, j_chunk,
+ * blocks_per_cu, variant, copies, layout; the arguments of sf_set_tuning).  Later calls with a
+ * matching key and no sf_set_tuning override use it.  (The reference's nearest counterpart picks
+ * a tile size from the problem shape, ext/cuda/kernels_2d.jl:171-187.) */
+int sf_autotune(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t out[6]);
 
 /* Plans: build once, execute many times.  A plan fixes the slices' shapes, bins, operator and
  * precision, and the device pointers x and u of every slice; the data behind those pointers may
@@ -201,14 +204,16 @@ int sf_check(int32_t n_slices, const sf_problem *slices, char *msg, int32_t msg_
 int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a, const double *b, double *out);
 
 /* Kernel tuning override (autotuner hook): threads per workgroup (= rows of an i-tile; a multiple
- * of 64 in 64..1024),
- * column-range length per work item, workgroups per CU, and the column-record variant of the
- * binary32 kernel (1: staged through LDS, 2: scalar loads, 3: B-strips -- 1D slices of one batch
- * that share positions run four fields per pair evaluation).  Zero keeps the built-in choice.
- * Applies to subsequent calls on this context.  (No reference counterpart; the reference's GPU
- * path takes workgroup_size as a keyword, KA ext :385-401.) */
+ * of 64 in 64..1024), column-range length per work item, workgroups per CU, the column-record
+ * variant of the binary32 kernel (1: staged through LDS, 2: scalar loads, 3: B-strips -- 1D slices
+ * of one batch that share positions run four fields per pair evaluation), the privatised copies of
+ * the LDS histogram (1D: lane groups, a power of two up to 64; joint: 1 or 2, by lane halves) and
+ * the joint histogram's slot layout (1: distance-bin rows of n_val slots at an odd stride; 2: rows
+ * at a stride of 1 mod 32 slots).  Zero keeps the built-in choice.  Applies to subsequent calls on
+ * this context.  (No reference counterpart; the reference's GPU path takes workgroup_size as a
+ * keyword, KA ext :385-401.) */
 int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu,
-                  int32_t variant);
+                  int32_t variant, int32_t copies, int32_t layout);
 
 #ifdef __cplusplus
 }

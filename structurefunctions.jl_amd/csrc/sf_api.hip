@@ -209,6 +209,39 @@ struct Plan {
 
 }  // namespace
 
+namespace {
+// Launch shape of the histogram kernel; 0 keeps the built-in choice.  block: threads per
+// workgroup (rows of an i-tile); j_chunk: columns per work item; bpc: workgroups per CU; variant:
+// column records through LDS (1), scalar loads (2) or B-strips (3); copies: privatised copies of
+// the LDS histogram (1D: lane groups, a power of two up to 64; joint: 1 or 2, by lane halves);
+// layout: joint histogram slots value-bin fastest (1) or distance-bin fastest (2), odd row stride.
+struct Tuning {
+  int block = 0, j_chunk = 0, bpc = 0, variant = 0, copies = 0, layout = 0;
+  bool any() const { return block || j_chunk || bpc || variant || copies || layout; }
+};
+
+// Autotuner key: (geometry, D, joint, precision code, operator class, 2D single-pass planes,
+// log2 of the pair count of the caller's slices, n_dist, n_val).  Shared by the tuner and the
+// plan builder so that an entry the tuner stores is the one a later call looks up.
+using TuneKey = std::tuple<int, int, int, int, int, int, int, int, int>;
+int hist_opk_of(const sf_problem &p0) {
+  Params Pk{};
+  Pk.op = p0.op;
+  Pk.nl = p0.nl;
+  Pk.nt = p0.nt;
+  Pk.n_val = p0.n_val;
+  Pk.nmom = (p0.op == SF_OP_SINGLE_PASS && p0.n_val == 0) ? 4 : 1;
+  return hist_opk(Pk);
+}
+TuneKey tune_key(const sf_problem &p0, int64_t input_pairs) {
+  int lg = 0;
+  for (int64_t v = input_pairs; v > 1; v >>= 1) ++lg;
+  const int G = p0.geometry == SF_GEOM_FLAT ? kFlat : kSphere;
+  return TuneKey(G, p0.D, p0.n_val > 0 ? 1 : 0, p0.precision, hist_opk_of(p0),
+                 sp2d(p0.op, p0.n_val) ? 1 : 0, lg, p0.n_dist, p0.n_val);
+}
+}  // namespace
+
 struct sf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -219,12 +252,11 @@ struct sf_ctx {
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool ev_pending = false;
-  int tune_block = 0, tune_jchunk = 0, tune_bpc = 0, tune_variant = 0;
+  Tuning tune;  // sf_set_tuning override (all zero: none)
   // largest re-bin queue a call of this context has needed (entries): later plans start there
   int64_t queue_hint = 0;
-  // autotuner cache: (geometry, D, joint, mixed, operator class, log2 pairs, n_dist, n_val) ->
-  // (block, j_chunk, blocks_per_cu, variant)
-  std::map<std::tuple<int, int, int, int, int, int, int, int>, std::tuple<int, int, int, int>> tuned;
+  // autotuner cache: the call's shape (tune_key) -> the fastest launch shape measured for it
+  std::map<TuneKey, Tuning> tuned;
   // device buffers: host-input staging and the plan reused by one-shot calls
   DevBuf<double> d_x, d_u;
   Plan oneshot;
@@ -643,16 +675,16 @@ extern "C" int sf_eval_math(sf_ctx *ctx, int32_t fn, int64_t n, const double *a,
 }
 
 extern "C" int sf_set_tuning(sf_ctx *ctx, int32_t block, int32_t j_chunk, int32_t blocks_per_cu,
-                             int32_t variant) {
+                             int32_t variant, int32_t copies, int32_t layout) {
   if (!ctx) return SF_E_ARG;
   if (block != 0 && (block < 64 || block > 1024 || block % 64 != 0))
     return ctx->fail(SF_E_ARG, "block must be 0 or a multiple of 64 in [64, 1024]");
   if (j_chunk < 0 || blocks_per_cu < 0) return ctx->fail(SF_E_ARG, "tuning values must be >= 0");
   if (variant < 0 || variant > 3) return ctx->fail(SF_E_ARG, "variant must be 0, 1, 2 or 3");
-  ctx->tune_variant = variant;
-  ctx->tune_block = block;
-  ctx->tune_jchunk = j_chunk;
-  ctx->tune_bpc = blocks_per_cu;
+  if (copies < 0 || copies > 64 || (copies & (copies - 1)) != 0)
+    return ctx->fail(SF_E_ARG, "copies must be 0 or a power of two up to 64");
+  if (layout < 0 || layout > 2) return ctx->fail(SF_E_ARG, "layout must be 0, 1 or 2");
+  ctx->tune = Tuning{block, j_chunk, blocks_per_cu, variant, copies, layout};
   return SF_OK;
 }
 
@@ -803,7 +835,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   // Opt-in (tuning variant 3): the per-slice 1D kernel with 16 histogram copies outruns it (eight
   // sphere fields, N = 65536: 828 against 609 Gpairs/s), its column velocities being LDS reads.
   pl.strip = 0;
-  if (ctx->tune_variant == 3 && p0.precision != SF_PREC_FP64 && p0.n_val == 0 &&
+  if (ctx->tune.variant == 3 && p0.precision != SF_PREC_FP64 && p0.n_val == 0 &&
       p0.op != SF_OP_SINGLE_PASS && n_in >= 2) {
     bool same = true;
     for (int s = 1; s < n && same; ++s)
@@ -886,15 +918,14 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   Pk.n_val = p0.n_val;
   Pk.nmom = pl.sp ? 4 : 1;
   const int nmom = Pk.nmom;
-  int t_block = ctx->tune_block, t_jchunk = ctx->tune_jchunk, t_bpc = ctx->tune_bpc;
-  int t_variant = ctx->tune_variant;
-  if (!t_block && !t_jchunk && !t_bpc && !t_variant) {
-    int lg = 0;
-    for (int64_t v = pl.total_pairs; v > 1; v >>= 1) ++lg;
-    const auto key = std::make_tuple(G, D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val);
-    const auto it = ctx->tuned.find(key);
-    if (it != ctx->tuned.end()) std::tie(t_block, t_jchunk, t_bpc, t_variant) = it->second;
+  Tuning tn = ctx->tune;
+  if (!tn.any()) {
+    int64_t in_pairs = 0;
+    for (int s = 0; s < n_in; ++s) in_pairs += pairs_of(sl_in[s].N);
+    const auto it = ctx->tuned.find(tune_key(sl_in[0], in_pairs));
+    if (it != ctx->tuned.end()) tn = it->second;
   }
+  const int t_block = tn.block, t_jchunk = tn.j_chunk, t_bpc = tn.bpc, t_variant = tn.variant;
   // fixed-point joint: 768 threads, two workgroups per CU; other joint: 1024 threads, one per CU
   // 1D single pass: 512 threads (its 36-byte slots fill the LDS at two or three workgroups per CU;
   // more waves per workgroup then pay: flat 2D 595 against 492 Gpairs/s, sphere 644 against 529).
@@ -917,13 +948,19 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     // per-lane-group copies of a 1D histogram (lanes l and l + copies share one): 16 measured
     // best on both binary32 1D kernels (flat 2D du_L^2 at N = 65536: 636 / 643 / 840 / 708 / 597
     // Gpairs/s for 64 / 32 / 16 / 8 / 4 copies; sphere at N = 262144: 601 / 605 / 890 / 732 / 610)
-    copies = 16;
+    copies = tn.copies ? tn.copies : 16;
     while (copies > 1 &&
            (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0, strip).total * 2 > ctx->max_smem)
       copies >>= 1;
+  } else if (mixed && tn.copies == 2) {
+    copies = 2;  // joint histogram privatised by lane halves (binary32 kernel only)
   }
-  const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0, strip);
-  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, copies, nmom, 0, 0);
+  const int jlayout = (J && tn.layout) ? tn.layout : 1;
+  const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0, strip,
+                                    jlayout);
+  // the binary64 kernel keeps one copy of a joint histogram
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, J ? 1 : copies, nmom, 0, 0, 0,
+                                     jlayout);
   if ((size_t)std::max(L0.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
                      "histogram of %d cells needs %d bytes of LDS; the device allows %zu per workgroup",
@@ -944,7 +981,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   int qcap = 0;
   if (mixed) {
     for (int q = kQueueCap; q >= 64 && qcap == 0; q >>= 1) {
-      const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q, strip).total;
+      const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q, strip,
+                                jlayout).total;
       if ((size_t)t <= ctx->max_smem && occupancy(block, (size_t)t) >= bpc) qcap = q;
     }
   }
@@ -983,10 +1021,16 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   for (int s = 0; s < n; ++s) order[s] = s;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sl[a].N > sl[b].N; });
   std::vector<Item> items, items64;
+  // fixed-point cells: a workgroup flushes its LDS histogram only between items, so one item may
+  // add at most kFxFlushLimit updates (block rows x its columns).  A culled item spans one column
+  // tile of `block` columns; at block 1024 that is 2^20 updates, so such tiles are split in halves
+  // (the tile of a column is still j0 / block).
+  int64_t jc_cull = block;
+  while (pl.fx && jc_cull > 1 && (int64_t)block * jc_cull > kFxFlushLimit) jc_cull >>= 1;
   for (int s : order) {
     const int64_t N = sl[s].N;
     const CullInfo &ci = cinfo[s];
-    const int64_t jc = ci.active ? block : jchunk;
+    const int64_t jc = ci.active ? jc_cull : jchunk;
     const size_t first = items.size();
     for (int64_t i0 = 0; i0 < N - 1; i0 += block) {
       for (int64_t j0 = i0; j0 < N; j0 += jc) {
@@ -1011,6 +1055,13 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     std::stable_partition(items.begin() + first, items.end(), [N, rows, jc](const Item &a) {
       return N - a.i0 >= rows && a.j1 - a.j0 == jc;
     });
+  }
+  if (pl.fx) {
+    // every item must fit the fixed-point cells' exact range (checked, not assumed)
+    for (const Item &it : items)
+      if ((int64_t)block * (int64_t)(it.j1 - it.j0) > kFxFlushLimit)
+        return ctx->fail(SF_E_DEVICE, "internal: work item of %d x %d pair updates exceeds the fixed-point flush limit",
+                         block, it.j1 - it.j0);
   }
   if (items.empty()) items.push_back(Item{0, 0, 0, 0});
   if (strip && items64.empty()) items64.push_back(Item{0, 0, 0, 0});
@@ -1077,6 +1128,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.ncell_max = ncell_max;
   P.tj = tj;
   P.copies = copies;
+  P.jlayout = jlayout;
   P.stage_bytes = stage_bytes;
   P.n_dist = p0.n_dist;
   P.n_val = p0.n_val;
@@ -1118,6 +1170,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   P.stage_bytes = stage;
   if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts, no queue
     P.fx = 0;
+    if (P.n_val > 0) P.copies = 1;
     P.qcap = 0;
     P.qw = 0;
     P.flush_limit = 0x7fffffffLL;
@@ -1130,7 +1183,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int stage_mixed = stage * ((use_mixed && pl.strip) ? pl.strip : 1);
   P.stage_bytes = stage_mixed;
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_mixed, P.copies, P.nmom, P.fx, P.qcap,
-                                   P.strip);
+                                   P.strip, P.jlayout);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
@@ -1425,7 +1478,7 @@ extern "C" void sf_plan_destroy(sf_plan *plan) {
 // ----------------------------------------------------------------------------------------------
 // autotuner
 // ----------------------------------------------------------------------------------------------
-extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t out[4]) {
+extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t out[6]) {
   if (!ctx) return SF_E_ARG;
   ctx->err.clear();
   int rc = validate_batch(ctx, n, sl, false);
@@ -1434,21 +1487,13 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
   HIPCHK(hipSetDevice(ctx->device));
   const sf_problem &p0 = sl[0];
   const bool J = p0.n_val > 0, mixed = p0.precision != SF_PREC_FP64;
+  const bool fx = p0.precision == SF_PREC_MIXED_FIXED && J;
   const int ncell = out_cells(p0);
   DevBuf<double> ts;
   DevBuf<unsigned long long> tc;
   HIPCHK(ts.ensure((size_t)n * ncell));
   HIPCHK(tc.ensure((size_t)n * ncell));
-  std::vector<std::tuple<int, int, int, int>> cand;
-  const int blocks[3] = {256, 512, 1024};
-  for (int b : blocks) {
-    if (J && b < 512) continue;
-    if (!J && b > 512) continue;
-    for (int v = (mixed ? 1 : 0); v <= (mixed ? 2 : 0); ++v) {
-      for (int jc_scale = 0; jc_scale < 2; ++jc_scale) cand.emplace_back(b, jc_scale ? 4 * b : 0, 0, v);
-    }
-  }
-  const int saved[4] = {ctx->tune_block, ctx->tune_jchunk, ctx->tune_bpc, ctx->tune_variant};
+  const Tuning saved = ctx->tune;
   std::vector<const double *> px(n), pu(n);
   std::vector<double> xmax(n, 0.0);
   for (int s = 0; s < n; ++s) {
@@ -1456,42 +1501,86 @@ extern "C" int sf_autotune(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t
     pu[s] = sl[s].u;
     xmax[s] = 1.0;  // timing only: margins do not change the work, results are discarded
   }
-  double best = 1e300;
-  std::tuple<int, int, int, int> best_c = cand.front();
-  for (const auto &c : cand) {
-    std::tie(ctx->tune_block, ctx->tune_jchunk, ctx->tune_bpc, ctx->tune_variant) = c;
-    double t = 0;
-    for (int rep = 0; rep < 2 && !rc; ++rep) {
-      rc = run_batch(ctx, n, sl, px.data(), pu.data(), xmax.data(), ts.p, tc.p, ctx->stream);
+  // device time of one call with launch shape c (a warm-up call first)
+  auto time_of = [&](const Tuning &c, double *t) {
+    ctx->tune = c;
+    for (int rep = 0; rep < 2; ++rep) {
+      const int r = run_batch(ctx, n, sl, px.data(), pu.data(), xmax.data(), ts.p, tc.p, ctx->stream);
+      if (r) return r;
       int64_t st[8];
       sf_last_stats(ctx, st);
-      t = (double)st[6];
+      *t = (double)st[6];
     }
-    if (rc) break;
+    return SF_OK;
+  };
+  // coordinate search: threads per workgroup x column variant, then the histogram's privatised
+  // copies and (joint) slot layout, then the column-range length
+  std::vector<Tuning> stage1;
+  const int blocks[4] = {256, 512, 768, 1024};
+  for (int b : blocks) {
+    if (J && b < 512) continue;
+    if (!J && b > 512) continue;
+    if (J && b == 768 && !fx) continue;  // 768: the fixed-point cells' two-workgroup launch
+    for (int v = (mixed ? 1 : 0); v <= (mixed ? 2 : 0); ++v) stage1.push_back(Tuning{b, 0, 0, v, 0, 0});
+  }
+  double best = 1e300;
+  Tuning best_c = stage1.front();
+  auto consider = [&](const Tuning &c) {
+    double t = 0;
+    const int r = time_of(c, &t);
+    if (r == SF_E_UNSUPPORTED) return SF_OK;  // shape does not fit this device: skip it
+    if (r) return r;
     if (t < best) {
       best = t;
       best_c = c;
     }
+    return SF_OK;
+  };
+  for (const Tuning &c : stage1)
+    if ((rc = consider(c))) break;
+  if (!rc) {
+    std::vector<Tuning> stage2;
+    if (!J) {
+      for (int cp : {4, 8, 32}) {
+        Tuning c = best_c;
+        c.copies = cp;
+        stage2.push_back(c);
+      }
+    } else if (mixed) {
+      for (int cp : {1, 2})
+        for (int lay : {1, 2}) {
+          if (cp == 1 && lay == 1) continue;  // the built-in choice, already timed
+          Tuning c = best_c;
+          c.copies = cp;
+          c.layout = lay;
+          stage2.push_back(c);
+        }
+    }
+    for (const Tuning &c : stage2)
+      if ((rc = consider(c))) break;
   }
-  ctx->tune_block = saved[0];
-  ctx->tune_jchunk = saved[1];
-  ctx->tune_bpc = saved[2];
-  ctx->tune_variant = saved[3];
+  if (!rc) {
+    const int b = best_c.block ? best_c.block : 256;
+    for (int sc : {2, 8}) {
+      Tuning c = best_c;
+      c.j_chunk = sc * b;
+      if ((rc = consider(c))) break;
+    }
+  }
+  ctx->tune = saved;
   ts.release();
   tc.release();
   if (rc) return rc;
   int64_t total = 0;
   for (int s = 0; s < n; ++s) total += pairs_of(sl[s].N);
-  int lg = 0;
-  for (int64_t v = total; v > 1; v >>= 1) ++lg;
-  Params Pk{};
-  Pk.op = p0.op;
-  Pk.nl = p0.nl;
-  Pk.nt = p0.nt;
-  Pk.n_val = p0.n_val;
-  Pk.nmom = p0.op == SF_OP_SINGLE_PASS && p0.n_val == 0 ? 4 : 1;
-  const int G = p0.geometry == SF_GEOM_FLAT ? kFlat : kSphere;
-  ctx->tuned[std::make_tuple(G, p0.D, (int)J, (int)mixed, hist_opk(Pk), lg, p0.n_dist, p0.n_val)] = best_c;
-  if (out) std::tie(out[0], out[1], out[2], out[3]) = best_c;
+  ctx->tuned[tune_key(p0, total)] = best_c;
+  if (out) {
+    out[0] = best_c.block;
+    out[1] = best_c.j_chunk;
+    out[2] = best_c.bpc;
+    out[3] = best_c.variant;
+    out[4] = best_c.copies;
+    out[5] = best_c.layout;
+  }
   return SF_OK;
 }

@@ -105,8 +105,8 @@ struct Params {
   unsigned long long *stats;  // [0] pairs visited by exact kernel recheck
   int32_t ncell_max;       // largest table of any slice (LDS sizing)
   int32_t tj;              // column records staged in LDS per chunk
-  int32_t copies;          // 1D: per-lane histogram copies (power of two); joint: 1
-  int32_t vstride;         // joint: LDS row stride of a distance bin (odd, >= n_val); 1D: 0
+  int32_t copies;          // 1D: per-lane histogram copies (power of two); joint: 1 or 2 (lane halves)
+  int32_t jlayout;         // joint: LDS slot layout (joint_layout below); 1D: 0
   int32_t stage_bytes;     // LDS bytes of the column stage
   int32_t n_dist, n_val;
   int32_t op, nl, nt;
@@ -140,10 +140,30 @@ struct SmemLayout {
   int32_t hs, hc, tab, dE, vE, vE32, stage, q, total;  // byte offsets
 };
 
-// LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each): joint histograms are stored value-bin fastest with an odd row stride per
-// distance bin (lanes of one wave usually share a distance bin but not a value bin, so their
-// atomics then fall in different banks); 1D histograms keep one copy per lane group.
-inline __host__ __device__ int joint_stride(int n_val) { return n_val | 1; }
+// LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each).  1D
+// histograms keep one copy per lane group.  A joint histogram's slot of (distance bin d, value bin
+// v), 0-based, is v * jv + d * jd within one copy of `slots` slots:
+//   layout 1 (default): distance-bin rows of n_val slots at an odd stride (lanes of one wave
+//                       usually share a distance bin but not a value bin, so their atomics then
+//                       fall in different banks): jv = 1, jd = n_val | 1;
+//   layout 2:           value-bin rows of n_dist slots at an odd stride: jd = 1, jv = n_dist | 1.
+// With two copies (privatisation by lane halves) lanes 32..63 use the second copy.
+struct JointLayout {
+  int32_t jv, jd, slots;
+};
+inline __host__ __device__ JointLayout joint_layout(int n_dist, int n_val, int layout) {
+  JointLayout L;
+  if (layout == 2) {
+    L.jv = n_dist | 1;
+    L.jd = 1;
+    L.slots = n_val * L.jv;
+  } else {
+    L.jv = 1;
+    L.jd = n_val | 1;
+    L.slots = n_dist * L.jd;
+  }
+  return L;
+}
 // single pass, 2D form (op SF_OP_SINGLE_PASS with value bins): every input slice runs as six
 // plane slices, one per invariant, each a joint histogram written to every sixth output cell
 inline __host__ __device__ bool sp2d(int op, int n_val) { return op == 6 && n_val > 0; }
@@ -161,14 +181,15 @@ inline constexpr int64_t kQueueMax = 1LL << 28;
 // velocity record in registers)
 inline constexpr int kStripMax = 4;
 
+// copies: 1D lane-group copies, or the joint histogram's copies (1 or 2); jlayout: joint layout
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
                                                   int stage_bytes, int copies, int nmom, int fx,
-                                                  int qcap, int strip = 0) {
+                                                  int qcap, int strip = 0, int jlayout = 1) {
   SmemLayout L;
   // + kDummySlots: one scratch slot per lane, target of unconditional atomics of lanes that have
   // nothing to add (never flushed); a B-strip launch keeps one histogram per field
-  const int nh = ((n_val > 0) ? n_dist * joint_stride(n_val) : n_dist * copies) * (strip > 1 ? strip : 1) +
-                 kDummySlots;
+  const int one = (n_val > 0) ? joint_layout(n_dist, n_val, jlayout).slots : n_dist;
+  const int nh = one * copies * (strip > 1 ? strip : 1) + kDummySlots;
   int o = 0;
   L.hs = o; o += 8 * nh * nmom;
   L.hc = o; o += fx ? 0 : 4 * ((nh + 3) & ~3);  // fixed-point cells carry their count

@@ -457,9 +457,11 @@ struct Smem {
   float *vE32;
   char *stage;
   int ncell_h;
-  int copies;
+  int copies;     // 1D: lane-group copies; joint: copies by lane halves (1 or 2)
   bool joint;
-  int nd, vstride, nslots;
+  int nd, nslots;
+  int jv, jd, jslots;  // joint slot of (d, v) in one copy: v jv + d jd; a copy holds jslots slots
+  int coff;            // this lane's copy offset (joint; slots)
   int nmom;     // sums per slot, moment-major: moment k of slot h at hs[k * mstride + h]
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
   bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
@@ -472,7 +474,7 @@ struct Smem {
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx,
-                                   P.qcap, P.strip);
+                                   P.qcap, P.strip, P.jlayout);
   Smem m;
   m.nmom = P.nmom;
   m.fx = P.fx != 0;
@@ -480,8 +482,12 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.copies = P.copies;
   m.joint = P.n_val > 0;
   m.nd = P.n_dist;
-  m.vstride = P.n_val > 0 ? joint_stride(P.n_val) : 0;
-  m.nslots = P.n_val > 0 ? P.n_dist * m.vstride : P.n_dist * P.copies;
+  const JointLayout JL = joint_layout(P.n_dist, P.n_val, P.jlayout);
+  m.jv = JL.jv;
+  m.jd = JL.jd;
+  m.jslots = P.n_val > 0 ? JL.slots : 0;
+  m.coff = (P.n_val > 0 && P.copies == 2) ? (int)(__lane_id() >> 5) * JL.slots : 0;
+  m.nslots = P.n_val > 0 ? JL.slots * P.copies : P.n_dist * P.copies;
   // B-strip: field b's histogram at slots [b nslots, (b + 1) nslots), scratch slots after all
   m.mstride = m.nslots * (P.strip > 1 ? P.strip : 1) + kDummySlots;
   m.hs = (double *)(base + L.hs);
@@ -509,7 +515,7 @@ DEV void zero_hist(const Smem &m) {
 
 // LDS slot of output cell c = db0 + n_dist * vb0 (0-based bins) for this lane
 DEV int hslot(const Smem &m, int db0, int vb0) {
-  if (m.joint) return vb0 + m.vstride * db0;
+  if (m.joint) return vb0 * m.jv + db0 * m.jd + m.coff;
   return db0 * m.copies + (__lane_id() & (m.copies - 1));
 }
 
@@ -537,12 +543,16 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
     // fixed-point joint cells: n = word >> 44, sum v = n a_g + (sum q - n kFxBias) / vsf
     for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
       const int db0 = c % m.nd, vb0 = c / m.nd;
-      const int h = vb0 + m.vstride * db0;
-      const unsigned long long wd = m.hq[h];
-      m.hq[h] = 0ull;
-      const unsigned long long n = wd >> kFxShift;
+      const int h0 = vb0 * m.jv + db0 * m.jd;
+      unsigned long long n = 0, sq = 0;
+      for (int k = 0; k < m.copies; ++k) {
+        const int h = h0 + k * m.jslots;
+        const unsigned long long wd = m.hq[h];
+        m.hq[h] = 0ull;
+        n += wd >> kFxShift;
+        sq += wd & ((1ull << kFxShift) - 1ull);
+      }
       if (n) {
-        const unsigned long long sq = wd & ((1ull << kFxShift) - 1ull);
         const double a = (double)__builtin_fmaf((float)vb0, s.vwf, s.v0f);
         const double v = (double)n * a + ((double)sq - (double)n * (double)kFxBias) / (double)s.vsf;
         const int64_t o = s.out_off + ostride * c;
@@ -581,11 +591,14 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
     double v = 0.0;
     if (m.joint) {
       const int db0 = c % m.nd, vb0 = c / m.nd;
-      const int h = vb0 + m.vstride * db0;
-      n = m.hc[h];
-      v = m.hs[h];
-      m.hc[h] = 0u;
-      m.hs[h] = 0.0;
+      const int h0 = vb0 * m.jv + db0 * m.jd;
+      for (int k = 0; k < m.copies; ++k) {
+        const int h = h0 + k * m.jslots;
+        n += m.hc[h];
+        v += m.hs[h];
+        m.hc[h] = 0u;
+        m.hs[h] = 0.0;
+      }
     } else {
       for (int k = 0; k < m.copies; ++k) {
         n += m.hc[c * m.copies + k];
@@ -1037,7 +1050,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   const bool add = upper & !amb & db_ok & vb_ok;
   const bool defer = upper & amb & (amb_d | db_ok);
   int h;
-  if constexpr (J) h = vb0 + m.vstride * (db - 1);
+  if constexpr (J) h = vb0 * m.jv + (db - 1) * m.jd + m.coff;
   else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
   pa.h = add ? h : -1;
   pa.v = SP ? L : v;

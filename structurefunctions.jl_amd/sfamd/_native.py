@@ -98,7 +98,7 @@ def lib():
     L.sf_check.restype = ctypes.c_int
     L.sf_eval_math.argtypes = [vp, i32, i64, dp, dp, dp]
     L.sf_eval_math.restype = ctypes.c_int
-    L.sf_set_tuning.argtypes = [vp, i32, i32, i32, i32]
+    L.sf_set_tuning.argtypes = [vp, i32, i32, i32, i32, i32, i32]
     L.sf_set_tuning.restype = ctypes.c_int
     L.sf_plan_create.argtypes = [vp, i32, ctypes.POINTER(SfProblem), ctypes.POINTER(vp)]
     L.sf_plan_create.restype = ctypes.c_int
@@ -171,13 +171,15 @@ class Context:
         return {"pairs": out[0], "rebinned": out[1], "items": out[2], "blocks": out[3],
                 "hist_ns": out[4], "recheck_ns": out[5], "call_ns": out[6], "block": out[7]}
 
-    def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0, variant=0):
-        self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu), int(variant)))
+    def set_tuning(self, block=0, j_chunk=0, blocks_per_cu=0, variant=0, copies=0, layout=0):
+        self.check(lib().sf_set_tuning(self._h, int(block), int(j_chunk), int(blocks_per_cu), int(variant),
+                                       int(copies), int(layout)))
 
     def autotune(self, problems):
-        """sf_autotune on device-pointer problems; returns (block, j_chunk, blocks_per_cu, variant)."""
+        """sf_autotune on device-pointer problems; returns (block, j_chunk, blocks_per_cu, variant,
+        copies, layout)."""
         arr = (SfProblem * len(problems))(*problems)
-        out = (ctypes.c_int32 * 4)()
+        out = (ctypes.c_int32 * 6)()
         self.check(lib().sf_autotune(self._h, len(problems), arr, out))
         return tuple(out)
 

@@ -91,8 +91,9 @@ def test_autotune_then_call_matches_fp64(ctx):
         p.precision, p.count_bits = prec, 64
         return p
 
-    block, jchunk, bpc, variant = ctx.autotune([prob(1)])
+    block, jchunk, bpc, variant, copies, layout = ctx.autotune([prob(1)])
     assert block in (256, 512, 768, 1024) and variant in (0, 1, 2) and jchunk >= 0
+    assert copies in (0, 1, 2) and layout in (0, 1, 2)
     outs = {}
     for prec in (0, 1):
         s = torch.zeros(nd * nv, dtype=torch.float64, device=dev)
