@@ -222,21 +222,27 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        import _oracle
-        threads = min(16, os.cpu_count() or 1)
-        # estimate the rate on a small row sample, then time a sample of about cpu_seconds
-        def time_rows(stride):
-            t = time.perf_counter()
-            _oracle.hist(x, u, dedges, op=(0, 1, 0), geometry="deg", vedges=vedges, nthreads=threads,
-                         compensated=False, rows=(0, N, stride))
-            return time.perf_counter() - t, sum(N - 1 - i for i in range(0, N, stride))
-        t_est, p_est = time_rows(4096)
-        rate = p_est / max(t_est, 1e-6)
-        stride = max(1, int(pairs / max(rate * args.cpu_seconds, 1.0)))
-        t_s, p_s = time_rows(stride)
-        cpu = {"value": round(p_s / t_s / 1e9, 4), "unit": "Gpairs/s", "cores": threads, "kind": "port",
-               "sample": f"every {stride}th outer row i of the same workload ({p_s} pairs), "
-                         f"C++ restatement of SerialBackend/ThreadedBackend, binary64, {threads} threads"}
+        host = host_info()
+        threads = host["threads"]
+
+        def oracle_rows(rows, nthreads):
+            import _oracle
+            _oracle.hist(x, u, dedges, op=(0, 1, 0), geometry="deg", vedges=vedges, nthreads=nthreads,
+                         compensated=False, rows=rows)
+
+        # ThreadedBackend restatement on the host's CPU share, SerialBackend on one thread; each
+        # on a row sample of about cpu_seconds (min of two timings, benchmark/regimes.jl:31-41)
+        thr = cpu_rate(oracle_rows, N, threads, args.cpu_seconds)
+        ser = cpu_rate(oracle_rows, N, 1, args.cpu_seconds)
+        cpu = {"value": round(thr["gpairs_s"], 4), "unit": "Gpairs/s", "cores": threads, "kind": "port",
+               "sample": f"every {thr['stride']}th outer row i of the same workload ({thr['pairs']} pairs), "
+                         f"C++ restatement of ThreadedBackend (round-robin rows, per-thread histograms), "
+                         f"binary64, {threads} threads ({host['cpu']}, nproc {host['nproc']})",
+               "serial_1_thread_gpairs_s": round(ser["gpairs_s"], 5),
+               "serial_sample": f"every {ser['stride']}th outer row ({ser['pairs']} pairs), SerialBackend restatement"}
+        # configuration 1 (the reference's CPU-runnable case) at its full size on one thread, and
+        # the same call on the GPU
+        cpu["cfg1_serial_full"] = cfg1_serial_vs_gpu(ctx, stream, dev)
 
     out = {
         "metric": METRIC,
@@ -260,6 +266,87 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out))
+
+
+def host_info():
+    """CPU model, visible processors, and the threads a baseline may use: the process's CPU share
+    (OMP_NUM_THREADS when the host sets it, else the affinity mask)."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else nproc
+    return {"cpu": model, "nproc": nproc, "threads": max(1, min(threads, nproc))}
+
+
+def cpu_rate(run_rows, N, nthreads, seconds):
+    """Pairs per second of run_rows((0, N, stride), nthreads) on a row sample sized to about
+    `seconds` (a short estimate first), min of two timings."""
+    def pairs_of(stride):
+        return sum(N - 1 - i for i in range(0, N, stride))
+
+    t0 = time.perf_counter()
+    est_stride = 4096
+    run_rows((0, N, est_stride), nthreads)
+    rate = pairs_of(est_stride) / max(time.perf_counter() - t0, 1e-6)
+    total = N * (N - 1) // 2
+    stride = max(1, int(total / max(rate * seconds / 2, 1.0)))
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        run_rows((0, N, stride), nthreads)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    p = pairs_of(stride)
+    return {"gpairs_s": p / best / 1e9, "stride": stride, "pairs": p}
+
+
+def cfg1_serial_vs_gpu(ctx, stream, dev):
+    """Configuration 1 (N = 4096 flat 2D, du_L^2 over 32 log bins): the SerialBackend restatement on
+    one thread over all 8,386,560 pairs, against one binary64 and one mixed device call."""
+    import numpy as np
+    import torch
+
+    import _oracle
+    from sfamd import _native, synthetic
+
+    x, u, e = synthetic.config1()
+    N = x.shape[1]
+    pairs = N * (N - 1) // 2
+    best = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        s_o, c_o, _ = _oracle.hist(x, u, e, op=(0, 2, 0), nthreads=1, compensated=False)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
+    du = torch.from_numpy(np.ascontiguousarray(u.T)).to(dev)
+    out = {"pairs": pairs, "serial_s": round(best, 4), "serial_gpairs_s": round(pairs / best / 1e9, 4)}
+    for prec in ("fp64", "mixed"):
+        s = torch.zeros(len(e) - 1, dtype=torch.float64, device=dev)
+        c = torch.zeros(len(e) - 1, dtype=torch.int64, device=dev)
+        p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, 2, 2, 0, (0, 2, 0), e, None, prec)
+        gbest = None
+        for r in range(3):
+            s.zero_()
+            c.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.hist_device([p], s.data_ptr(), c.data_ptr(), stream)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if r:
+                gbest = dt if gbest is None else min(gbest, dt)
+        out[f"gpu_{prec}_ms"] = round(gbest * 1e3, 4)
+        out[f"gpu_{prec}_counts_equal_serial"] = bool(np.array_equal(c.cpu().numpy().astype(np.uint64), c_o))
+    return out
 
 
 def _device_problem(_native, xd, ud, N, D, W, geom, op, dedges, vedges, prec):
@@ -583,6 +670,25 @@ def bench_batch(args):
     torch.cuda.synchronize()
     rate_b64 = pairs / (time.perf_counter() - t0) / 1e9
 
+    # ThreadedBackend restatement on the host's CPU share: both outputs of every 16th slice
+    cpu = None
+    if not args.no_cpu:
+        import _oracle
+        host = host_info()
+        sample = list(range(0, S, 16))
+        t0 = time.perf_counter()
+        for k in sample:
+            _oracle.hist(xs[k], us[k], dedges, op=(0, 2, 0), nthreads=host["threads"], compensated=False)
+            _oracle.hist(xs[k], us[k], dedges, op=(0, 1, 0), vedges=vbins[k], nthreads=host["threads"],
+                         compensated=False)
+        dt_cpu = time.perf_counter() - t0
+        p_cpu = sum(Ns[k] * (Ns[k] - 1) // 2 for k in sample)
+        cpu = {"value": round(p_cpu / dt_cpu / 1e9, 4), "unit": "Gpairs/s", "cores": host["threads"],
+               "kind": "port",
+               "sample": f"slices {sample[0]}, {sample[1]}, ... (every 16th, {p_cpu} pairs, both outputs), "
+                         f"C++ restatement of ThreadedBackend, binary64, {host['threads']} threads "
+                         f"({host['cpu']}, nproc {host['nproc']})"}
+
     out = {"metric": "Gpairs/s, batched ragged time slices (N(N-1)/2 summed over slices per second; "
                      "two outputs per slice)",
            "value": round(rate, 3), "pair_evaluations_per_s_G": round(2 * rate, 3), "unit": "Gpairs/s", "n_gpus": 1, "steps": args.steps,
@@ -592,7 +698,8 @@ def bench_batch(args):
                       "slices": S, "pairs_per_operator": pairs, "plan_build_s": round(t_build, 3)},
            "sequential_fp64_gpairs_s": round(rate_seq, 3),
            "batched_fp64_gpairs_s": round(rate_b64, 3),
-           "speedup_vs_sequential_fp64": round(rate / rate_seq, 3)}
+           "speedup_vs_sequential_fp64": round(rate / rate_seq, 3),
+           "cpu_baseline": cpu}
     del keep
     print(json.dumps(out))
 

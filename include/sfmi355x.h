@@ -93,6 +93,28 @@ extern "C" {
 
 typedef struct sf_ctx sf_ctx;
 
+/* A bin axis given as a LinearBinEdges (src/BinEdges.jl:94-131), optionally inside
+ * InfPaddedBinEdges (:270-345).  The reference digitizes such an axis with an O(1) rule rather
+ * than a search of the edge values:
+ *   searchsortedfirst(v, x) = 1 if x <= first_edge; n + 1 if x > last_edge; otherwise
+ *     idx = clamp(floor(muladd(x, inv_step, -first_edge * inv_step)) + 1, 1, n),
+ *     idx + 1 if muladd(idx - 1, step_val, first_edge) < x, else idx  (BinEdges.jl:114-131),
+ * and on the flat geometry's squared-distance plan (SquaredLinearPlan, :412-417, 506, 516) the
+ * key is sqrt(r^2).  Its result can differ from a search of the edge values for queries within a
+ * few ulps of an edge; passing the rule makes the counts those of the reference.  muladd is taken
+ * as a fused multiply-add (what Julia emits on x86-64 with FMA and on AArch64).  The fields are
+ * the LinearBinEdges fields; the edge vector of the problem must hold the same axis materialised
+ * (collect(range), with -Inf / +Inf around it when padded) and is used for sizes and for the
+ * binary32 kernels' certified binning.  A NaN query, which the reference rejects (floor(Int, NaN)
+ * throws), is not counted. */
+#define SF_RULE_LINEAR 1
+typedef struct {
+  int32_t kind;               /* SF_RULE_LINEAR                                                   */
+  int32_t padded;             /* 1: InfPaddedBinEdges around the range (catch-all first/last bins) */
+  int64_t n;                  /* length of the range (edges of the unpadded axis)                 */
+  double first_edge, last_edge, inv_step, step_val;  /* LinearBinEdges fields (BinEdges.jl:94-100) */
+} sf_bin_rule;
+
 typedef struct {
   int32_t D;                  /* velocity components: 2 or 3                                     */
   int32_t W;                  /* coordinate rows of x: D (flat) or 2 (sphere)                     */
@@ -113,6 +135,9 @@ typedef struct {
                                  dispatch.jl:69); calls whose N(N-1)/2 exceeds it are rejected    */
   int32_t semantics;          /* SF_SEM_*: which of the reference's pair loops the results follow */
   int32_t reserved;           /* 0                                                                */
+  const sf_bin_rule *dist_rule;  /* NULL: distance bins are searched in dist_edges (BinEdges /
+                                    LogBinEdges' physical edges); else the LinearBinEdges rule     */
+  const sf_bin_rule *val_rule;   /* the same for the value axis (NULL for the 2D single pass)      */
 } sf_problem;
 
 /* Context: device, stream, persistent device workspace.

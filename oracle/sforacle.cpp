@@ -47,6 +47,57 @@ inline int64_t count_isless(const double *v, int64_t n, double x) {
   return lo;
 }
 
+// searchsortedfirst(::LinearBinEdges, x) (BinEdges.jl:114-131).  Julia's muladd is taken as a
+// fused multiply-add; -f * inv_step is a separate product.
+int64_t linear_ssf(const sfo_bin_rule &r, double x) {
+  const double f = r.first_edge;
+  if (x <= f) return 1;
+  const double l = r.last_edge;
+  const int64_t n = r.n;
+  if (x > l) return n + 1;
+  const double t = fma(x, r.inv_step, -f * r.inv_step);
+  int64_t idx = (int64_t)floor(t) + 1;
+  if (idx < 1) idx = 1;
+  if (idx > n) idx = n;
+  const double u = fma((double)(idx - 1), r.step_val, f);
+  return u < x ? idx + 1 : idx;
+}
+
+// digitize(x, edges) = searchsortedfirst(edges, x) - 1 (HelperFunctions.jl:77-84) for a
+// LinearBinEdges axis, or one inside InfPaddedBinEdges (BinEdges.jl:306-316: x <= -Inf -> 1,
+// x > last -> length + 2, else the inner result + 1).  The reference rejects NaN
+// (floor(Int, NaN) throws); here it lands past the last bin, i.e. it is not counted.
+int64_t rule_digitize(const sfo_bin_rule &r, double x) {
+  const int64_t nb = r.padded ? r.n + 1 : r.n - 1;
+  if (x != x) return nb + 1;
+  if (r.padded) {
+    if (x <= -__builtin_inf()) return 0;
+    if (x > r.last_edge) return r.n + 1;
+    return linear_ssf(r, x);
+  }
+  return linear_ssf(r, x) - 1;
+}
+
+// distance bin of a flat pair on the point-field kernels: the squared-distance plan
+// (BinEdges.jl:410-558): searchsortedfirst(sqedges, r^2) - 1, or with a LinearBinEdges axis
+// (SquaredLinearPlan) the rule on sqrt(r^2)
+inline int64_t flat_point_bin(const sfo_problem *p, const double *sqedges, double r2) {
+  if (p->dist_rule) return rule_digitize(*p->dist_rule, sqrt(r2));
+  return count_isless(sqedges, p->n_dist + 1, r2);
+}
+
+// digitize(d, distance_bins) of a distance d (flat batch-leading r, or the spherical distance)
+inline int64_t dist_bin(const sfo_problem *p, double d) {
+  if (p->dist_rule) return rule_digitize(*p->dist_rule, d);
+  return count_isless(p->dist_edges, p->n_dist + 1, d);
+}
+
+// digitize(v, value_bins)
+inline int64_t val_bin(const sfo_problem *p, double v) {
+  if (p->val_rule) return rule_digitize(*p->val_rule, v);
+  return count_isless(p->val_edges, p->n_val + 1, v);
+}
+
 struct Op {
   int op, nl, nt, D;
 };
@@ -269,6 +320,13 @@ int validate(const sfo_problem *p) {
   if (p->geometry == SFO_GEOM_FLAT)
     for (int k = 0; k <= p->n_dist; ++k)
       if (p->dist_edges[k] < 0.0 && !(k == 0 && p->dist_edges[0] == -__builtin_inf())) return -8;
+  // a LinearBinEdges rule must give the axis' bin count
+  if (p->dist_rule && (p->dist_rule->n < 2 ||
+                       p->n_dist != (p->dist_rule->padded ? p->dist_rule->n + 1 : p->dist_rule->n - 1)))
+    return -8;
+  if (p->val_rule && (p->val_rule->n < 2 ||
+                      p->n_val != (p->val_rule->padded ? p->val_rule->n + 1 : p->val_rule->n - 1)))
+    return -9;
   if (p->n_val < 0) return -9;
   if (p->n_val > 0) {
     if (!p->val_edges) return -9;
@@ -306,18 +364,17 @@ void row(const Prob &P, int64_t i, Acc &acc) {
         double rh[3] = {0.0, 0.0, 0.0};
         for (int k = 0; k < D; ++k) rh[k] = dx[k] / r;
         val = op_apply(P.o, du, rh);
-        b = count_isless(p->dist_edges, nb + 1, r);
+        b = dist_bin(p, r);
       } else {
         val = sf_raw(P.o, du, dx, r2);
-        // squared_bin(SquaredGeneralPlan) = searchsortedfirst(sqedges, r2) - 1
-        b = count_isless(P.sqedges.data(), nb + 1, r2);
+        b = flat_point_bin(p, P.sqedges.data(), r2);
       }
       if (1 <= b && b <= nb) {
         const double mag = mag_of(du, D, P.order);
         if (nv == 0) {
           acc.add((size_t)(b - 1), val, mag);
         } else {
-          const int64_t vb = count_isless(p->val_edges, nv + 1, val);
+          const int64_t vb = val_bin(p, val);
           if (1 <= vb && vb <= nv) acc.add((size_t)((b - 1) + (int64_t)nb * (vb - 1)), val, mag);
         }
       }
@@ -330,7 +387,7 @@ void row(const Prob &P, int64_t i, Acc &acc) {
   const double rh[3] = {1.0, 0.0, 0.0};  // pair_direction(::SphericalGeometry{D}) = e1
   for (int64_t j = i + 1; j < N; ++j) {
     const SpherePair f = sphere_frame(pA, &P.P[3 * j], EA, NA, &P.E[3 * j], &P.Nn[3 * j], p->radius);
-    const int64_t b = count_isless(p->dist_edges, nb + 1, f.distance);
+    const int64_t b = dist_bin(p, f.distance);
     if (f.ok && 1 <= b && b < nb + 1) {
       double du[3];
       sphere_delta(f, uA, &p->u[D * j], D, du);
@@ -339,7 +396,7 @@ void row(const Prob &P, int64_t i, Acc &acc) {
       if (nv == 0) {
         acc.add((size_t)(b - 1), val, mag);
       } else {
-        const int64_t vb = count_isless(p->val_edges, nv + 1, val);
+        const int64_t vb = val_bin(p, val);
         if (1 <= vb && vb < nv + 1) acc.add((size_t)((b - 1) + (int64_t)nb * (vb - 1)), val, mag);
       }
     }
@@ -422,6 +479,8 @@ extern "C" int sfo_hist(const sfo_problem *p, int nthreads, int compensated, int
   return 0;
 }
 
+extern "C" int64_t sfo_rule_digitize(const sfo_bin_rule *r, double x) { return rule_digitize(*r, x); }
+
 extern "C" void sfo_sincosd(double x, double *s, double *c) { sf_sincosd(x, s, c); }
 extern "C" void sfo_sincos(double x, double *s, double *c) { sf_sincos(x, s, c); }
 extern "C" double sfo_atan2(double y, double x) { return sf_atan2(y, x); }
@@ -445,7 +504,7 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
       double rh[3] = {0.0, 0.0, 0.0};
       for (int k = 0; k < D; ++k) rh[k] = dx[k] / *distance;
       *value = op_apply(o, du, rh);
-      *dbin = count_isless(p->dist_edges, p->n_dist + 1, *distance);
+      *dbin = dist_bin(p, *distance);
     } else {
       *value = sf_raw(o, du, dx, r2);
       std::vector<double> sq(p->n_dist + 1);
@@ -453,9 +512,9 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
         const double e = p->dist_edges[k];
         sq[k] = (e == -__builtin_inf()) ? e : e * e;
       }
-      *dbin = count_isless(sq.data(), p->n_dist + 1, r2);
+      *dbin = flat_point_bin(p, sq.data(), r2);
     }
-    if (p->n_val > 0) *vbin = count_isless(p->val_edges, p->n_val + 1, *value);
+    if (p->n_val > 0) *vbin = val_bin(p, *value);
     return 1;
   }
   double Pa[3], Ea[3], Na[3], Pb[3], Eb[3], Nb[3];
@@ -467,8 +526,8 @@ extern "C" int sfo_pair(const sfo_problem *p, int64_t i, int64_t j, double *dist
   const double rh[3] = {1.0, 0.0, 0.0};
   *distance = f.distance;
   *value = op_apply(o, du, rh);
-  *dbin = count_isless(p->dist_edges, p->n_dist + 1, f.distance);
-  if (p->n_val > 0) *vbin = count_isless(p->val_edges, p->n_val + 1, *value);
+  *dbin = dist_bin(p, f.distance);
+  if (p->n_val > 0) *vbin = val_bin(p, *value);
   return f.ok ? 1 : 0;
 }
 
@@ -530,18 +589,18 @@ extern "C" int sfo_hist_single_pass(const sfo_problem *p, int nthreads, int64_t 
             double rh[3] = {0.0, 0.0, 0.0};
             for (int d = 0; d < D; ++d) rh[d] = dx[d] / r;
             duL = sdot(du, rh, D);
-            b = count_isless(q.dist_edges, nb + 1, r);
+            b = dist_bin(&q, r);
           } else {
             const double inv_r = 1.0 / sqrt(r2);
             duL = sdot(du, dx, D) * inv_r;
-            b = count_isless(P.sqedges.data(), nb + 1, r2);
+            b = flat_point_bin(&q, P.sqedges.data(), r2);
           }
           dn2 = sdot(du, du, D);
           if (!(1 <= b && b <= nb)) continue;
         } else {
           const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],
                                             &P.E[3 * j], &P.Nn[3 * j], q.radius);
-          b = count_isless(q.dist_edges, nb + 1, f.distance);
+          b = dist_bin(&q, f.distance);
           if (!(f.ok && 1 <= b && b <= nb)) continue;
           double du[3];
           sphere_delta(f, &q.u[D * i], &q.u[D * j], D, du);
@@ -661,11 +720,11 @@ extern "C" int sfo_hist_single_pass_2d(const sfo_problem *p, int nthreads, int64
             double rh[3] = {0.0, 0.0, 0.0};
             for (int d = 0; d < D; ++d) rh[d] = dx[d] / r;
             duL = sdot(du, rh, D);
-            b = count_isless(q.dist_edges, nb + 1, r);
+            b = dist_bin(&q, r);
           } else {
             const double inv_r = 1.0 / sqrt(r2);
             duL = sdot(du, dx, D) * inv_r;
-            b = count_isless(P.sqedges.data(), nb + 1, r2);
+            b = flat_point_bin(&q, P.sqedges.data(), r2);
           }
           dn2 = sdot(du, du, D);
           scale = sqrt(dn2);
@@ -673,7 +732,7 @@ extern "C" int sfo_hist_single_pass_2d(const sfo_problem *p, int nthreads, int64
         } else {
           const SpherePair f = sphere_frame(&P.P[3 * i], &P.P[3 * j], &P.E[3 * i], &P.Nn[3 * i],
                                             &P.E[3 * j], &P.Nn[3 * j], q.radius);
-          b = count_isless(q.dist_edges, nb + 1, f.distance);
+          b = dist_bin(&q, f.distance);
           if (!(f.ok && 1 <= b && b <= nb)) continue;
           double du[3];
           sphere_delta(f, &q.u[D * i], &q.u[D * j], D, du);

@@ -28,6 +28,18 @@ extern "C" {
 #define SFO_OP_L1T2COMP 4       /* LongitudinalTransverseComponentThirdOrder...Type    :145,150 */
 #define SFO_OP_FULLVEC 5        /* FullVectorStructureFunctionType{nl} (nl = NF)       :114,125 */
 
+/* A LinearBinEdges axis (src/BinEdges.jl:94-131), optionally inside InfPaddedBinEdges
+ * (:270-345): the fields of the reference's struct.  With a rule the axis is digitized by the
+ * reference's O(1) rule (searchsortedfirst(::LinearBinEdges), :114-131) instead of a search of
+ * the edge vector; the flat squared-distance plan applies it to sqrt(r^2) (SquaredLinearPlan,
+ * :412-417, 506, 516). */
+typedef struct {
+  int32_t kind;           /* 1: LinearBinEdges                                         */
+  int32_t padded;         /* InfPaddedBinEdges around the range                        */
+  int64_t n;              /* length of the range                                       */
+  double first_edge, last_edge, inv_step, step_val;
+} sfo_bin_rule;
+
 typedef struct {
   int32_t D;              /* velocity components, 2 or 3                               */
   int32_t W;              /* coordinate width: D (flat) or 2 (sphere lon, lat)         */
@@ -45,7 +57,12 @@ typedef struct {
   int32_t reserved;       /* (fields with auxiliary axes, batch_leading.jl:105-330):
                              r = sqrt(dx.dx) binned against the edges, values from the
                              operator on rhat = dx / r; 0: the point-field kernels     */
+  const sfo_bin_rule *dist_rule;  /* NULL: search dist_edges                              */
+  const sfo_bin_rule *val_rule;   /* NULL: search val_edges                               */
 } sfo_problem;
+
+/* digitize(x, rule) = searchsortedfirst - 1 for one query (tests of the rule itself) */
+int64_t sfo_rule_digitize(const sfo_bin_rule *r, double x);
 
 /* Histogram over outer rows i = i_begin, i_begin + i_step, ... < i_end (0-based), pairs (i, j>i).
  * nthreads > 1 splits those rows round-robin over threads with per-thread histograms reduced by

@@ -284,6 +284,26 @@ struct sf_ctx {
 
 namespace {
 
+// A LinearBinEdges rule must describe the materialised edge vector it comes with: n edges (plus
+// -Inf / +Inf when padded), first and last edges equal to its fields, a finite non-negative step.
+int validate_rule(sf_ctx *ctx, const sf_bin_rule *r, int nbins, const double *edges, const char *what) {
+  if (r->kind != SF_RULE_LINEAR) return ctx->fail(SF_E_ARG, "%s: unknown bin rule kind %d", what, r->kind);
+  if (r->padded != 0 && r->padded != 1) return ctx->fail(SF_E_ARG, "%s: padded must be 0 or 1", what);
+  const int64_t want = r->padded ? r->n + 1 : r->n - 1;
+  if (r->n < 2 || want != nbins)
+    return ctx->fail(SF_E_DIM, "%s: LinearBinEdges of length %lld does not give %d bins", what, (long long)r->n, nbins);
+  if (!(r->first_edge > -INFINITY && r->last_edge < INFINITY && r->first_edge <= r->last_edge))
+    return ctx->fail(SF_E_ARG, "%s: LinearBinEdges needs finite first <= last", what);
+  if (!(r->step_val >= 0.0 && r->step_val < INFINITY) || r->inv_step != r->inv_step)
+    return ctx->fail(SF_E_ARG, "%s: LinearBinEdges needs a finite non-negative step", what);
+  const int o = r->padded ? 1 : 0;
+  if (edges[o] != r->first_edge || edges[o + r->n - 1] != r->last_edge)
+    return ctx->fail(SF_E_ARG, "%s: the edge vector does not hold the LinearBinEdges range", what);
+  if (r->padded && !(edges[0] == -INFINITY && edges[nbins] == INFINITY))
+    return ctx->fail(SF_E_ARG, "%s: a padded rule needs -Inf and +Inf around the range", what);
+  return SF_OK;
+}
+
 int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
   if (!p) return ctx->fail(SF_E_ARG, "slice %d: null problem", idx);
   if (p->D != 2 && p->D != 3)
@@ -360,6 +380,17 @@ int validate_one(sf_ctx *ctx, const sf_problem *p, bool host_ptrs, int idx) {
     return ctx->fail(SF_E_ARG, "count_bits must be 32 or 64");
   if (p->semantics != SF_SEM_POINT && p->semantics != SF_SEM_BATCH)
     return ctx->fail(SF_E_ARG, "semantics must be SF_SEM_POINT or SF_SEM_BATCH");
+  if (p->dist_rule) {
+    const int rc = validate_rule(ctx, p->dist_rule, p->n_dist, p->dist_edges, "distance_bins");
+    if (rc) return rc;
+  }
+  if (p->val_rule) {
+    if (p->n_val == 0) return ctx->fail(SF_E_ARG, "val_rule given without value_bins");
+    if (p->op == SF_OP_SINGLE_PASS)
+      return ctx->fail(SF_E_UNSUPPORTED, "the single pass's 2D form takes explicit value edge vectors only");
+    const int rc = validate_rule(ctx, p->val_rule, p->n_val, p->val_edges, "value_bins");
+    if (rc) return rc;
+  }
   const unsigned __int128 pairs = (unsigned __int128)p->N * (unsigned __int128)(p->N > 0 ? p->N - 1 : 0) / 2;
   const unsigned __int128 cmax = p->count_bits == 32 ? 0xffffffffull : 0xffffffffffffffffull;
   if (pairs > cmax)
@@ -398,6 +429,25 @@ struct SliceTables {
   std::vector<double> dedges;  // exact keys
   Slice desc{};
 };
+
+// binary64 decision points of a LinearBinEdges rule lie within this distance of the edge values
+long double rule_slack(const sf_bin_rule *r) {
+  if (!r) return 0.0L;
+  return ldexpl(1.0L, -48) * (fabsl((long double)r->first_edge) + fabsl((long double)r->last_edge));
+}
+
+LinRule lin_rule_of(const sf_bin_rule *r) {
+  LinRule L{};
+  if (!r) return L;
+  L.on = 1;
+  L.padded = r->padded;
+  L.n = r->n;
+  L.f = r->first_edge;
+  L.l = r->last_edge;
+  L.inv = r->inv_step;
+  L.step = r->step_val;
+  return L;
+}
 
 void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   const int nE = p.n_dist + 1;
@@ -450,10 +500,14 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   // margins: relative 2^-18 (64 ulp) plus an absolute term for the hi/lo split of the inputs
   const long double drel = ldexpl(1.0L, -20);
   const long double e1 = flat ? ldexpl(1.0L, -44) * std::max((long double)xmax, 1e-300L) : ldexpl(1.0L, -44);
+  // a LinearBinEdges rule decides within a few binary64 ulps of max(|first|, |last|) of each
+  // materialised edge (its rounding of the step, 1 / step and the two multiply-adds); the margin
+  // covers 2^-48 (|first| + |last|), in key-space units
+  const long double dr = rule_slack(p.dist_rule) / R;
   std::vector<float> lo(nE), hi(nE);
   for (int k = 0; k < nE; ++k) {
     if (C[k] == INFINITY) { lo[k] = INFINITY; hi[k] = INFINITY; continue; }
-    const long double a = 2.0L * sqrtl(fabsl(C[k])) * e1 + e1 * e1 + 1e-30L;
+    const long double a = 2.0L * sqrtl(fabsl(C[k])) * (e1 + dr) + (e1 + dr) * (e1 + dr) + 1e-30L;
     lo[k] = round_down_f(C[k] - fabsl(C[k]) * drel - a);
     hi[k] = round_up_f(C[k] + fabsl(C[k]) * drel + a);
   }
@@ -490,6 +544,8 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
     }
   }
   Slice &s = T.desc;
+  s.drule = lin_rule_of(p.dist_rule);
+  s.vrule = p.n_val > 0 ? lin_rule_of(p.val_rule) : LinRule{};
   s.M64 = P64.M;
   s.ncell64 = n64;
   s.base64 = ((int64_t)(P64.elo + 1023) << P64.M);  // cell 0 starts at 2^elo
@@ -499,7 +555,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   // value axis
   s.v_uniform = 0;
   s.v_negzero = 0;
-  s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0;
+  s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0; s.vabs = 0;
   s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0;
   s.v_log = 0; s.vk0 = 0; s.vl0f = 0; s.vlinvf = 0;
   if (p.n_val > 0) {
@@ -519,7 +575,8 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
         s.vinvf = (float)(1.0L / w);
         // slack in t units: edge non-uniformity + rounding of v0 and 1/w in binary32
         const long double u = ldexpl(1.0L, -24);
-        s.vslack = round_up_f(dev + 4.0L * u * (1.0L + fabsl((long double)v0 / w)) + 4.0L * u * nv + 1e-6L);
+        s.vslack = round_up_f(dev + 4.0L * u * (1.0L + fabsl((long double)v0 / w)) + 4.0L * u * nv + 1e-6L +
+                              rule_slack(p.val_rule) / w);
         // fixed-point joint cells need every value of bin g within [-1/64, 1 + 1/64] widths of
         // the binary32 grid point fmaf(g, vwf, v0f)
         if (dev < 1.0L / 64) {
@@ -530,6 +587,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
       }
     }
     if (!s.v_uniform) s.vslack = (float)ldexpl(1.0L, -21);  // relative slack for binary32 edges
+    s.vabs = round_up_f(2.0L * rule_slack(p.val_rule));
     if (!s.v_uniform) {
       // log-spaced from the first positive edge on (a leading 0 or negative edges allowed): the
       // binary32 kernel guesses the bin from log2(v) and walks to it; the walk alone decides, so
@@ -840,7 +898,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     bool same = true;
     for (int s = 1; s < n && same; ++s)
       same = sl[s].x == p0.x && sl[s].N == p0.N && xmax[s] == xmax[0] &&
-             memcmp(sl[s].dist_edges, p0.dist_edges, sizeof(double) * (p0.n_dist + 1)) == 0;
+             memcmp(sl[s].dist_edges, p0.dist_edges, sizeof(double) * (p0.n_dist + 1)) == 0 &&
+             (sl[s].dist_rule == nullptr) == (p0.dist_rule == nullptr) &&
+             (!p0.dist_rule || memcmp(sl[s].dist_rule, p0.dist_rule, sizeof(sf_bin_rule)) == 0);
     if (same) pl.strip = kStripMax;
   }
   const int strip = pl.strip;

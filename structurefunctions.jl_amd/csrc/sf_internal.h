@@ -44,7 +44,17 @@ struct Cell {
   int32_t bx;     // binary64 path: number of exact edges below the cell's lower bound
 };
 
+// Binary64 O(1) bin rule of a LinearBinEdges axis (optionally InfPadded); see lin_rule_bin.
+struct LinRule {
+  double f, l, inv, step;  // first_edge, last_edge, inv_step, step_val
+  int64_t n;               // length of the range
+  int32_t on;              // 0: search the edge vector instead
+  int32_t padded;          // InfPaddedBinEdges around the range
+};
+
 struct Slice {
+  LinRule drule;      // distance axis rule (flat point-field kernels: applied to sqrt(r^2))
+  LinRule vrule;      // value axis rule
   int64_t N;          // points
   int64_t pt_off;     // first record of this slice in the record arrays
   int64_t out_off;    // first output cell
@@ -61,6 +71,7 @@ struct Slice {
   double v0, vinv;    // value guess t = (v - v0) * vinv
   float v0f, vinvf;   // binary32 value guess
   float vslack;       // binary32 value-axis slack in units of t (edge non-uniformity + rounding)
+  float vabs;         // binary32 value-axis absolute slack of searched edges (LinearBinEdges rule)
   int32_t v_fx;       // value edges uniform to 1/64 of a bin: fixed-point joint accumulation allowed
   // fixed-point joint accumulation: a value v of value bin g (0-based) is stored as the integer
   // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
@@ -122,13 +133,19 @@ struct Params {
   int32_t strip;           // B-strip launch: LDS histograms per workgroup (fields per strip), 0: off
 };
 
-// Fixed-point joint cell: one 64-bit word, count in bits [44, 64), sum of q in bits [0, 44).
-// q < 2^23 (value within [-1/64, 1 + 1/64] bin widths of a_g, scaled by 2^22, biased by 2^20), so
-// fewer than 2^20 updates between flushes keep both fields exact.
+// Fixed-point joint cell: one 64-bit word, the sum over its updates of kFxUnit + q, where
+// kFxUnit = 2^44 + kFxBase and q < 2^23 (value within [-1/64, 1 + 1/64] bin widths of a_g, scaled
+// by 2^22, biased by 2^20).  With n updates the word is n kFxUnit + sum q and sum q < 2^43 <
+// kFxUnit, so n = word / kFxUnit and sum q = word - n kFxUnit; the word stays below 2^64 while
+// n <= kFxFlushLimit = 2^20 - 256 (2^20 kFxUnit alone would exceed it).
 inline constexpr int kFxShift = 44;
 inline constexpr float kFxScale = 4194304.0f;  // 2^22 per value-bin width
 inline constexpr float kFxBias = 1048576.0f;   // 2^20
-inline constexpr int64_t kFxFlushLimit = (1LL << 20) - 1;
+// the pair loop adds q as the bits of the binary32 number 2^23 + kFxBias + (value offset): that is
+// kFxBase + q; one update adds kFxUnit + q to the word (count 1 in bits 44.., kFxBase below)
+inline constexpr unsigned long long kFxBase = 0x4B000000ull;  // bits of 8388608.0f
+inline constexpr unsigned long long kFxUnit = (1ull << 44) + kFxBase;
+inline constexpr int64_t kFxFlushLimit = (1LL << 20) - 256;
 
 // Single-pass output: (6, n_dist) per slice, rows S2 L2 T2 S3 L3 L1T2; the kernels accumulate the
 // moments S2, L2, S3, L3 (rows 0, 1, 3, 4) and the count (row 0).

@@ -141,18 +141,30 @@ struct SphereFrame {
   bool ok;
 };
 
-DEV SphereFrame sphere_frame64(const double *p, const double *q, double radius) {
+// Everything of the frame but the central angle: d = q - p, p.d, sin^2 = |p x d|^2, ok.
+DEV SphereFrame sphere_frame64_noangle(const double *p, const double *q) {
   SphereFrame f;
   f.d[0] = q[0] - p[0]; f.d[1] = q[1] - p[1]; f.d[2] = q[2] - p[2];
-  const double s[3] = {p[0] + q[0], p[1] + q[1], p[2] + q[2]};
   f.dp = sdot<3>(f.d, p);
   const double *d = f.d;
   const double w[3] = {p[1] * d[2] - p[2] * d[1], p[2] * d[0] - p[0] * d[2],
                        p[0] * d[1] - p[1] * d[0]};
   f.s2 = sdot<3>(w, w);
-  const double sigma = 2.0 * sf_atan2(__builtin_sqrt(sdot<3>(d, d)), __builtin_sqrt(sdot<3>(s, s)));
   f.ok = f.s2 > 2.2250738585072014e-308;  // floatmin(Float64)
-  f.distance = radius * sigma;
+  f.distance = 0.0;
+  return f;
+}
+
+// distance = radius * 2 atan2(|d|, |p + q|), the reference's central angle
+DEV double sphere_distance64(const double *p, const double *q, const double *d, double radius) {
+  const double s[3] = {p[0] + q[0], p[1] + q[1], p[2] + q[2]};
+  const double sigma = 2.0 * sf_atan2(__builtin_sqrt(sdot<3>(d, d)), __builtin_sqrt(sdot<3>(s, s)));
+  return radius * sigma;
+}
+
+DEV SphereFrame sphere_frame64(const double *p, const double *q, double radius) {
+  SphereFrame f = sphere_frame64_noangle(p, q);
+  f.distance = sphere_distance64(p, q, f.d, radius);
   return f;
 }
 
@@ -205,6 +217,28 @@ DEV int dist_bin64(double key, const Cell *tab, const double *E, int nE, const S
   return b;
 }
 
+// digitize(x, ::LinearBinEdges) = searchsortedfirst(v, x) - 1 (BinEdges.jl:114-131), or inside
+// InfPaddedBinEdges (:306-316): the histogram bin, valid 1 .. nb (0 and nb + 1 are out of range).
+// muladd is fused.  A NaN query (the reference's floor(Int, NaN) throws) is out of range.
+DEV int lin_rule_bin(const LinRule &r, double x, int nb) {
+  if (x != x) return nb + 1;
+  if (r.padded && x <= -__builtin_inf()) return 0;
+  int64_t k;
+  if (x <= r.f) {
+    k = 1;
+  } else if (x > r.l) {
+    k = r.n + 1;
+  } else {
+    const double c = -r.f * r.inv;  // a separate product, as in the reference
+    const double t = __builtin_fma(x, r.inv, c);
+    int64_t idx = (int64_t)__builtin_floor(t) + 1;
+    idx = idx < 1 ? 1 : (idx > r.n ? r.n : idx);
+    const double u = __builtin_fma((double)(idx - 1), r.step, r.f);
+    k = u < x ? idx + 1 : idx;
+  }
+  return (int)(r.padded ? k : k - 1);
+}
+
 // searchsortedfirst(value_edges, v) - 1 with Julia isless semantics
 DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
   if (v != v) return nE;
@@ -253,9 +287,19 @@ DEV T sp_invariant(int t, T duL, T dn2) {
   }
 }
 
-template <int G, int D, bool J, int OPK = 0>
+DEV int cell32(float key, const Slice &s);
+
+// CERT (sphere): the distance bin is read from the chord^2 cell table of the binary32 kernel
+// (lohi / blo: ambiguity interval and bin count per cell) with the binary64 chord^2 |q - p|^2,
+// which is the reference's own |d|^2; the reference's central angle 2 atan2(|d|, |p + q|) is
+// monotone in it, and the table's margins (2^-20 relative and more) exceed the error of both the
+// binary32 rounding of the key and the reference's angle.  Only a pair whose chord^2 lies in a
+// cell's ambiguity interval computes the angle and searches (tab is then the global binary64 key
+// table).  Values are always the reference's operation sequence.
+template <int G, int D, bool J, int OPK = 0, bool CERT = false>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
-                   const Cell *tab, const double *dE, const double *vE, double *val) {
+                   const Cell *tab, const double *dE, const double *vE, double *val,
+                   const float2 *lohi = nullptr, const int *blo_tab = nullptr) {
   const int nEd = P.n_dist + 1;
   int db;
   double v;
@@ -272,7 +316,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       // against the edges themselves, rhat = dx / r (pair_direction), values from the operator
       // on (du, rhat), single-pass du_L = dot(du, rhat) (HelperFunctions.jl:248-330)
       const double r = __builtin_sqrt(r2);
-      db = dist_bin64(r, tab, dE, nEd, s);
+      db = s.drule.on ? lin_rule_bin(s.drule, r, P.n_dist) : dist_bin64(r, tab, dE, nEd, s);
       if (db < 1 || db > P.n_dist) return -1;
       double rh[D];
 #pragma unroll
@@ -297,11 +341,13 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       }
       *val = v;
       if (!J) return db - 1;
-      const int vb = val_bin64(v, vE, P.n_val + 1, s);
+      const int vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, vE, P.n_val + 1, s);
       if (vb < 1 || vb > P.n_val) return -1;
       return (db - 1) + P.n_dist * (vb - 1);
     }
-    db = dist_bin64(r2, tab, dE, nEd, s);
+    // squared-distance plan: binary search of r^2 in the squared edges, or (SquaredLinearPlan,
+    // BinEdges.jl:506,516) the linear rule on sqrt(r^2)
+    db = s.drule.on ? lin_rule_bin(s.drule, __builtin_sqrt(r2), P.n_dist) : dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
     if constexpr (OPK == 3 || OPK == 4) {
       const double inv_r = 1.0 / __builtin_sqrt(r2);
@@ -327,8 +373,21 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       v = sf_raw64<D>(P.op, P.nl, P.nt, du, dx, r2);
     }
   } else {
-    const SphereFrame f = sphere_frame64(A, B, P.radius);
-    db = dist_bin64(f.distance, tab, dE, nEd, s);
+    SphereFrame f;
+    bool exact_dist = true;
+    if constexpr (CERT) {
+      f = sphere_frame64_noangle(A, B);
+      const float key = (float)sdot<3>(f.d, f.d);
+      const int c = cell32(key, s);
+      const float2 lh = lohi[c];
+      exact_dist = (key >= lh.x) & (key <= lh.y);
+      db = (key > lh.y) ? blo_tab[c] + 1 : blo_tab[c];
+      if (exact_dist) f.distance = sphere_distance64(A, B, f.d, P.radius);
+    } else {
+      f = sphere_frame64(A, B, P.radius);
+    }
+    if (exact_dist)
+      db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist) : dist_bin64(f.distance, tab, dE, nEd, s);
     if (!f.ok || db < 1 || db > P.n_dist) return -1;
     double du[3];
     sphere_delta64<D>(f, A, B, du);
@@ -349,7 +408,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
   }
   *val = v;
   if (!J) return db - 1;
-  const int vb = val_bin64(v, vE, P.n_val + 1, s);
+  const int vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, vE, P.n_val + 1, s);
   if (vb < 1 || vb > P.n_val) return -1;
   return (db - 1) + P.n_dist * (vb - 1);
 }
@@ -462,6 +521,8 @@ struct Smem {
   int nd, nslots;
   int jv, jd, jslots;  // joint slot of (d, v) in one copy: v jv + d jd; a copy holds jslots slots
   int coff;            // this lane's copy offset (joint; slots)
+  int hoff;            // coff - jd: slot of (d, v) for a 1-based distance bin d is v jv + d jd + hoff
+  int scr;             // this lane's scratch slot (target of updates that add nothing)
   int nmom;     // sums per slot, moment-major: moment k of slot h at hs[k * mstride + h]
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
   bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
@@ -487,9 +548,11 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.jd = JL.jd;
   m.jslots = P.n_val > 0 ? JL.slots : 0;
   m.coff = (P.n_val > 0 && P.copies == 2) ? (int)(__lane_id() >> 5) * JL.slots : 0;
+  m.hoff = m.coff - m.jd;
   m.nslots = P.n_val > 0 ? JL.slots * P.copies : P.n_dist * P.copies;
   // B-strip: field b's histogram at slots [b nslots, (b + 1) nslots), scratch slots after all
   m.mstride = m.nslots * (P.strip > 1 ? P.strip : 1) + kDummySlots;
+  m.scr = m.nslots * (P.strip > 1 ? P.strip : 1) + (int)__lane_id();
   m.hs = (double *)(base + L.hs);
   m.hq = (unsigned long long *)(base + L.hs);
   m.hc = (unsigned *)(base + L.hc);
@@ -549,8 +612,11 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
         const int h = h0 + k * m.jslots;
         const unsigned long long wd = m.hq[h];
         m.hq[h] = 0ull;
-        n += wd >> kFxShift;
-        sq += wd & ((1ull << kFxShift) - 1ull);
+        // each update added kFxUnit = 2^44 + kFxBase plus its q < 2^23; fewer than 2^20 updates
+        // keep sum q below kFxUnit, so the quotient is the count
+        const unsigned long long nk = wd / kFxUnit;
+        n += nk;
+        sq += wd - nk * kFxUnit;
       }
       if (n) {
         const double a = (double)__builtin_fmaf((float)vb0, s.vwf, s.v0f);
@@ -655,7 +721,11 @@ DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const doubl
       rj[2 * k + 1] = q.y;
     }
     double v[2];
-    const int c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
+    int c;
+    if constexpr (G == kSphere)
+      c = exact_cell<G, D, J, OPK, true>(P, s, ri, rj, P.cells + s.tab_off, m.dE, m.vE, v, m.lohi, m.blo);
+    else
+      c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
     if (c >= 0) {
       const int h = hslot(m, c % P.n_dist, c / P.n_dist);
       if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
@@ -691,7 +761,9 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
       if (w.slice != cur) {
         cur = w.slice;
         s = P.slices[cur];
-        load_tables(m, P, s, false);
+        // sphere: the chord^2 table of certified distance bins (the binary64 key table is read
+        // from global memory by the rare pairs that need it)
+        load_tables(m, P, s, G == kSphere);
       }
       since_flush = 0;
     }
@@ -739,11 +811,14 @@ DEV int cell32(float key, const Slice &s) {
   return c;
 }
 
-// value-axis bin from a binary32 value with absolute error bound dv; sets amb when undecidable.
-// Returns the 0-based value bin g (valid bins 0 .. nE-2; anything else is out of range) and g as
-// a float in gf.
+
+// value-axis bin from a binary32 value with absolute error bound dv = dva * dvb; sets amb when
+// undecidable.  For uniform edges (VU) dva carries the factor 1 / bin width (the bound in bin
+// widths is then one multiply-add away).  Returns the 0-based value bin g (valid bins 0 .. nE-2;
+// anything else is out of range) and g as a float in gf.
 template <bool VU>
-DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, bool &amb, float &gf) {
+DEV int val_bin32(float v, float dva, float dvb, const Slice &s, const float *E32, int nE, bool &amb,
+                  float &gf) {
   // a non-finite value is undecidable here; the lookup below still runs, its bin is ignored
   amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   if constexpr (VU) {  // uniform value edges (s.v_uniform; chosen once per column range)
@@ -753,7 +828,7 @@ DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, b
     // |t|).  Undecidable iff an integer lies in (t - eps, t + eps].  Far outside the edges
     // (|t| >= 2^21) eps exceeds 1, so such values are undecidable too (and re-binned); the
     // float-to-int conversion saturates, so no clamp is needed.
-    const float eps = __builtin_fmaf(dv, s.vinvf, __builtin_fmaf(4.8e-7f, __builtin_fabsf(t), s.vslack));
+    const float eps = __builtin_fmaf(dva, dvb, __builtin_fmaf(4.8e-7f, __builtin_fabsf(t), s.vslack));
     const float gl = __builtin_floorf(t - eps), gh = __builtin_floorf(t + eps);
     amb = amb || (gl != gh);
     gf = gh;
@@ -776,7 +851,7 @@ DEV int val_bin32(float v, float dv, const Slice &s, const float *E32, int nE, b
     }
   }
   // nearest edges must be farther than the error bound (edges rounded to binary32: slack)
-  const float gap = dv + s.vslack * __builtin_fabsf(v) + 1e-37f;
+  const float gap = __builtin_fmaf(dva, dvb, __builtin_fmaf(s.vslack, __builtin_fabsf(v), s.vabs)) + 1e-37f;
   if (lo > 0 && v - E32[lo - 1] <= gap) amb = true;
   if (lo < nE && E32[lo] - v <= gap) amb = true;
   gf = (float)(lo - 1);
@@ -893,10 +968,10 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const int c = cell32(key, s);
     const float2 lh = m.lohi[c];
     const int blo = m.blo[c];
-    // the previous pair's update, unconditional (a lane with nothing pending adds to its scratch
-    // slot) so that the wait below is a partial one on every path
+    // the previous pair's update, unconditional (a lane with nothing pending holds its scratch
+    // slot in pa.h) so that the wait below is a partial one on every path
     __builtin_amdgcn_sched_barrier(0);
-    const int hh = pa.h >= 0 ? pa.h : m.nslots + (int)__lane_id();
+    const int hh = pa.h;
     if constexpr (FX) atomicAdd(&m.hq[hh], (1ull << kFxShift) | (unsigned long long)pa.q);
     else if constexpr (SP) hist_add_sp(m, hh, pa.v, pa.w);
     else hist_add(m, hh, pa.v);
@@ -1026,22 +1101,28 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // 2D single-pass planes: coefficient 32 * order, order 2 for |du|^2, du_L^2, du_T^2 and 3
     // for the products with du_L (the difference du_T^2 = |du|^2 - du_L^2 and the products add
     // the errors of both factors, hence twice the coefficient of one operator)
-    float dv;
+    // the bound is dva * ampv (with uniform value edges, in units of the bin width)
+    float dva;
     if constexpr (SP2) {
       const float u2 = mm * mm;
-      dv = (s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * mm) * ampv;
+      dva = s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * mm;
+      if constexpr (VU) dva *= s.vinvf;
+    } else if constexpr (VU) {
+      dva = (P.vcoef * s.vinvf) * pow_order<OPK>(P, mm);
     } else {
-      dv = P.vcoef * pow_order<OPK>(P, mm) * ampv;
+      dva = P.vcoef * pow_order<OPK>(P, mm);
     }
     // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
     amb = amb || !(umax >= 0x1p-30f);
     float gf;
-    vb0 = val_bin32<VU>(v, dv, s, m.vE32, P.n_val + 1, amb, gf);
+    vb0 = val_bin32<VU>(v, dva, ampv, s, m.vE32, P.n_val + 1, amb, gf);
     vb_ok = (unsigned)vb0 < (unsigned)P.n_val;
     if constexpr (FX) {
       // offset within the value bin in units of 2^-22 bin widths, biased, rounded to nearest
+      // kFxBias + 2^23 puts the result in [2^23, 2^24), where binary32 has unit spacing: the
+      // fma rounds to the nearest integer and the bits are kFxBase + q (no conversion needed)
       const float a = __builtin_fmaf(gf, s.vwf, s.v0f);
-      pa.q = (unsigned)__builtin_rintf(__builtin_fmaf(v - a, s.vsf, kFxBias));
+      pa.q = __float_as_uint(__builtin_fmaf(v - a, s.vsf, kFxBias + 8388608.0f));
     }
   } else if constexpr (!SP) {
     amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
@@ -1050,9 +1131,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   const bool add = upper & !amb & db_ok & vb_ok;
   const bool defer = upper & amb & (amb_d | db_ok);
   int h;
-  if constexpr (J) h = vb0 * m.jv + (db - 1) * m.jd + m.coff;
+  // 24-bit multiplies (full rate): every operand of a slot that is used is a small bin index;
+  // values of a pair that is not added may be garbage and are selected away below
+  if constexpr (J) h = __mul24(vb0, m.jv) + __mul24(db, m.jd) + m.hoff;
   else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
-  pa.h = add ? h : -1;
+  pa.h = add ? h : m.scr;
   pa.v = SP ? L : v;
   pa.w = n2;
   // scalar-load variant: an empty asm that reads the last three words of the next record makes
@@ -1182,7 +1265,7 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
   // cp walks the column records two at a time; the loads use constant offsets from it
   const float4 *cp = col + (j_begin - col_j0) * R;
   load_rec<F>(cp, 0, ra);
-  PendingAdd pa{-1, 0.f, 0.f, 0u};
+  PendingAdd pa{m.scr, 0.f, 0.f, 0u};
   int j = j_begin;
   // groups of 32 columns: bit t of dm flags pair (i, jg + t) for the re-bin queue
   while (j < j_end) {
@@ -1203,11 +1286,9 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
     }
     queue_deferred(P, m, dm, i, jg, slice_id);
   }
-  if (pa.h >= 0) {
-    if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
-    else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
-    else hist_add(m, pa.h, pa.v);
-  }
+  if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
+  else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
+  else hist_add(m, pa.h, pa.v);
 }
 
 // the value-edge lookup (uniform or searched) is chosen once per column range, not per pair

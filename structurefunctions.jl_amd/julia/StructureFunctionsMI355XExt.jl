@@ -51,6 +51,17 @@ end
 MI355XBackend(; device::Integer = 0, precision::Symbol = :fp64) = MI355XBackend(device, precision)
 
 # ---- C ABI mirror (include/sfmi355x.h) -----------------------------------------------------
+# sf_bin_rule: the fields of a LinearBinEdges (BinEdges.jl:94-100), optionally inside InfPaddedBinEdges
+struct SfBinRule
+    kind::Int32      # 1 = SF_RULE_LINEAR
+    padded::Int32
+    n::Int64
+    first_edge::Float64
+    last_edge::Float64
+    inv_step::Float64
+    step_val::Float64
+end
+
 struct SfProblem
     D::Int32
     W::Int32
@@ -70,6 +81,8 @@ struct SfProblem
     count_bits::Int32
     semantics::Int32   # SF_SEM_POINT (point-field kernels) here; SF_SEM_BATCH for auxiliary axes
     reserved::Int32
+    dist_rule::Ptr{SfBinRule}  # C_NULL: the distance edges are searched
+    val_rule::Ptr{SfBinRule}   # C_NULL: the value edges are searched
 end
 
 const _CONTEXTS = Dict{Int, Ptr{Cvoid}}()
@@ -113,25 +126,53 @@ _edges(b::SF.LogBinEdges) = SF.physical_edges_vector(b)
 _edges(b::SF.InfPaddedBinEdges) = vcat(-Inf, collect(Float64, b.edges), Inf)
 _edges(b::AbstractVector) = collect(Float64, b)
 
+# A LinearBinEdges axis (a range is wrapped as one, BinEdges.jl:363) is digitized by its O(1)
+# rule (BinEdges.jl:114-131; SquaredLinearPlan's sqrt(r^2) key on the flat path, :506, 516); the
+# library applies the same rule to these exact fields.  Other axes are searched (nothing).
+_linrule(b::SF.LinearBinEdges{Float64}, padded) =
+    Ref(SfBinRule(Int32(1), Int32(padded), Int64(length(b.edges)), b.first_edge, b.last_edge,
+                  b.inv_step, b.step_val))
+_linrule(b::SF.LinearBinEdges, padded) =
+    throw(ArgumentError("MI355XBackend digitizes LinearBinEdges in Float64; got $(eltype(b)) edges"))
+_rule(b::SF.LinearBinEdges) = _linrule(b, 0)
+_rule(b::AbstractRange) = _rule(SF.LinearBinEdges(b))
+_rule(b::SF.InfPaddedBinEdges) = b.edges isa SF.LinearBinEdges ? _linrule(b.edges, 1) : nothing
+_rule(b) = nothing
+_rptr(::Nothing) = Ptr{SfBinRule}(C_NULL)
+_rptr(r::Base.RefValue{SfBinRule}) = Base.unsafe_convert(Ptr{SfBinRule}, r)
+
+# The reference accumulates in OT = promote_type(float(eltype(x)), float(eltype(u)))
+# (src/Calculations/batch.jl:177-178) and runs its pair arithmetic in the inputs' type; the
+# library computes in binary64 only, so other element types are refused rather than silently
+# computed differently.
+function _require_float64(x, u)
+    eltype(x) === Float64 && eltype(u) === Float64 && return nothing
+    throw(ArgumentError("MI355XBackend computes in Float64: x and u must be Float64 arrays " *
+                        "(got $(eltype(x)), $(eltype(u)))"))
+end
+
 _count_bits(::Type{CT}) where {CT} = sizeof(CT) <= 4 ? Int32(32) : Int32(64)
 _prec(b::MI355XBackend) = b.precision === :mixed ? Int32(1) : b.precision === :mixed_fixed ? Int32(2) : Int32(0)
 
 function _run!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
                dbins, vbins, metric, CT)
+    _require_float64(x, u)
     ctx = _context(b.device)
     xc = convert(Matrix{Float64}, x)
     uc = convert(Matrix{Float64}, u)
     de = _edges(dbins)
     ve = vbins === nothing ? Float64[] : _edges(vbins)
+    dr = _rule(dbins)
+    vr = vbins === nothing ? nothing : _rule(vbins)
     op, nl, nt = _opcode(sf)
     geom, radius = _geometry(metric)
-    GC.@preserve xc uc de ve sums counts begin
+    GC.@preserve xc uc de ve dr vr sums counts begin
         p = SfProblem(Int32(size(uc, 1)), Int32(size(xc, 1)), Int64(size(xc, 2)),
                       pointer(xc), pointer(uc), geom, radius, op, nl, nt,
                       Int32(length(de) - 1), pointer(de),
                       Int32(vbins === nothing ? 0 : length(ve) - 1),
                       vbins === nothing ? Ptr{Float64}(C_NULL) : pointer(ve),
-                      _prec(b), _count_bits(CT), Int32(0), Int32(0))
+                      _prec(b), _count_bits(CT), Int32(0), Int32(0), _rptr(dr), _rptr(vr))
         rc = ccall((:sf_hist, libsfmi355x), Cint,
                    (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
                    ctx, p, sums, counts)
@@ -202,6 +243,7 @@ end
 # reference's batch results.
 function _run_aux!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, sf, x, u,
                    dbins, vbins, metric, CT)
+    _require_float64(x, u)
     ctx = _context(b.device)
     D, N = size(u, 1), size(u, 2)
     B = prod(size(u)[3:end])
@@ -211,16 +253,18 @@ function _run_aux!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt6
     W = size(xc, 1)
     de = _edges(dbins)
     ve = vbins === nothing ? Float64[] : _edges(vbins)
+    dr = _rule(dbins)
+    vr = vbins === nothing ? nothing : _rule(vbins)
     op, nl, nt = _opcode(sf)
     geom, radius = _geometry(metric)
-    GC.@preserve xc uc de ve sums counts begin
+    GC.@preserve xc uc de ve dr vr sums counts begin
         probs = [SfProblem(Int32(D), Int32(W), Int64(N),
                            shared ? pointer(xc) : pointer(xc, 1 + (k - 1) * W * N),
                            pointer(uc, 1 + (k - 1) * D * N), geom, radius, op, nl, nt,
                            Int32(length(de) - 1), pointer(de),
                            Int32(vbins === nothing ? 0 : length(ve) - 1),
                            vbins === nothing ? Ptr{Float64}(C_NULL) : pointer(ve),
-                           _prec(b), _count_bits(CT), Int32(1), Int32(0)) for k in 1:B]
+                           _prec(b), _count_bits(CT), Int32(1), Int32(0), _rptr(dr), _rptr(vr)) for k in 1:B]
         rc = ccall((:sf_hist_batch, libsfmi355x), Cint,
                    (Ptr{Cvoid}, Int32, Ptr{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
                    ctx, Int32(B), probs, sums, counts)
@@ -332,17 +376,20 @@ _sp2d_vector(value_bins, t) = value_bins isa Tuple ? value_bins[t] : value_bins
 
 function _run_sp2d!(b::MI355XBackend, sums::Vector{Float64}, counts::Vector{UInt64}, x, u,
                     dbins, value_bins, n_val::Int, metric, CT)
+    _require_float64(x, u)
     ctx = _context(b.device)
     xc = convert(Matrix{Float64}, x)
     uc = convert(Matrix{Float64}, u)
     de = _edges(dbins)
+    # the six value axes are searched (the library takes explicit vectors for this form)
     ve = reduce(vcat, (_edges(_sp2d_vector(value_bins, t))[1:(n_val + 1)] for t in 1:6))
+    dr = _rule(dbins)
     geom, radius = _geometry(metric)
-    GC.@preserve xc uc de ve sums counts begin
+    GC.@preserve xc uc de ve dr sums counts begin
         p = SfProblem(Int32(size(uc, 1)), Int32(size(xc, 1)), Int64(size(xc, 2)),
                       pointer(xc), pointer(uc), geom, radius, Int32(6), Int32(0), Int32(0),
                       Int32(length(de) - 1), pointer(de), Int32(n_val), pointer(ve),
-                      _prec(b), _count_bits(CT), Int32(0), Int32(0))
+                      _prec(b), _count_bits(CT), Int32(0), Int32(0), _rptr(dr), _rptr(nothing))
         rc = ccall((:sf_hist, libsfmi355x), Cint,
                    (Ptr{Cvoid}, Ref{SfProblem}, Ptr{Float64}, Ptr{UInt64}),
                    ctx, p, sums, counts)

@@ -28,6 +28,21 @@ EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batc
             "sf_autotune")
 
 
+class SfBinRule(ctypes.Structure):
+    """sf_bin_rule: a LinearBinEdges axis (kind 1), optionally InfPadded."""
+    _fields_ = [("kind", ctypes.c_int32), ("padded", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("first_edge", ctypes.c_double), ("last_edge", ctypes.c_double),
+                ("inv_step", ctypes.c_double), ("step_val", ctypes.c_double)]
+
+
+def make_rule(fields):
+    """SfBinRule from the dict of sfamd.bins.rule_of (None stays None)."""
+    if fields is None:
+        return None
+    return SfBinRule(1, 1 if fields["padded"] else 0, int(fields["n"]), float(fields["first_edge"]),
+                     float(fields["last_edge"]), float(fields["inv_step"]), float(fields["step_val"]))
+
+
 class SfProblem(ctypes.Structure):
     _fields_ = [
         ("D", ctypes.c_int32), ("W", ctypes.c_int32), ("N", ctypes.c_int64),
@@ -38,6 +53,7 @@ class SfProblem(ctypes.Structure):
         ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
         ("precision", ctypes.c_int32), ("count_bits", ctypes.c_int32),
         ("semantics", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("dist_rule", ctypes.c_void_p), ("val_rule", ctypes.c_void_p),
     ]
 
 

@@ -14,10 +14,12 @@ gfx950 library through its C ABI; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from . import _native
-from .bins import as_edges
+from .bins import as_edges, rule_of
 from .errors import ArgumentError, DimensionMismatch
 from .metrics import Euclidean, geometry_of
 from .objects import (StructureFunction, StructureFunction2DSumsAndCounts,
@@ -110,7 +112,17 @@ def _assert_counts_representable(count_dtype, n_points: int):
             f"{n_pairs} for N={n_points} (typemax = {tmax}); pass count_eltype=UInt64 or Int64.")
 
 
-def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep, semantics=_native.SEM_POINT):
+def _attach_rules(p, drule, vrule, keep):
+    """Point the problem at its axes' LinearBinEdges rules (None: searched axis)."""
+    for name, fields in (("dist_rule", drule), ("val_rule", vrule)):
+        r = _native.make_rule(fields)
+        if r is not None:
+            keep.append(r)
+            setattr(p, name, ctypes.addressof(r))
+
+
+def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep, semantics=_native.SEM_POINT,
+             drule=None, vrule=None):
     """Build one SfProblem for a (W, N) / (D, N) slice; keep references alive in `keep`."""
     geom, radius, _ = geometry_of(metric)
     xc = np.asfortranarray(x2, dtype=np.float64)
@@ -133,6 +145,7 @@ def _problem(sf, x2, u2, dedges, vedges, metric, precision, keep, semantics=_nat
     p.precision = precision
     p.count_bits = 64
     p.semantics = semantics
+    _attach_rules(p, drule, vrule, keep)
     return p
 
 
@@ -155,10 +168,14 @@ def _semantics(kind):
     return _native.SEM_POINT if kind == "point" else _native.SEM_BATCH
 
 
-def _run(sf, slices, dedges_list, vedges_list, metric, backend, semantics=_native.SEM_POINT):
+def _run(sf, slices, dedges_list, vedges_list, metric, backend, semantics=_native.SEM_POINT,
+         drules=None, vrules=None):
     keep = []
-    problems = [_problem(sf, xs, us, de, ve, metric, backend.precision_code, keep, semantics)
-                for (xs, us), de, ve in zip(slices, dedges_list, vedges_list)]
+    n = len(slices)
+    drules = drules or [None] * n
+    vrules = vrules or [None] * n
+    problems = [_problem(sf, xs, us, de, ve, metric, backend.precision_code, keep, semantics, dr, vr)
+                for (xs, us), de, ve, dr, vr in zip(slices, dedges_list, vedges_list, drules, vrules)]
     nd = len(dedges_list[0]) - 1
     sp = sf.code[0] == _native.OP_SINGLE_PASS
     if vedges_list[0] is None:
@@ -215,9 +232,11 @@ def calculate_structure_function(sf, x, u, distance_bins, value_bins=None, count
     _assert_counts_representable(count_eltype, x.shape[1])
     de = as_edges(distance_bins)
     ve = None if value_bins is None else as_edges(value_bins)
+    dr = rule_of(distance_bins)
+    vr = None if value_bins is None else rule_of(value_bins)
     slices, aux = _slices(x, u, kind)
     sums, counts, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend,
-                                _semantics(kind))
+                                _semantics(kind), [dr] * len(slices), [vr] * len(slices))
     B = len(slices)
     if ve is None:
         s = sums.reshape(B, nd).T.reshape((nd,) + aux, order="F")
@@ -250,8 +269,10 @@ def calculate_structure_function_inplace(sums, counts, sf, x, u, distance_bins, 
     slices, aux = _slices(x, u, kind)
     if tuple(np.shape(sums)) != want + aux or tuple(np.shape(counts)) != want + aux:
         raise DimensionMismatch(f"sums/counts must have shape {want + aux}; got {np.shape(sums)}, {np.shape(counts)}")
+    dr = rule_of(distance_bins)
+    vr = None if value_bins is None else rule_of(value_bins)
     s, c, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend,
-                        _semantics(kind))
+                        _semantics(kind), [dr] * len(slices), [vr] * len(slices))
     B = len(slices)
     ncell = nd * max(nv, 1)
     s = np.stack([s[b * ncell:(b + 1) * ncell].reshape(want, order="F") for b in range(B)], axis=-1)
@@ -278,13 +299,13 @@ def calculate_structure_function_batch(sf, xs, us, distance_bins, value_bins=Non
 
     def per_slice(b):
         if b is None:
-            return [None] * n
-        if isinstance(b, (list, tuple)) and len(b) == n and np.ndim(b[0]) == 1:
-            return [as_edges(e) for e in b]
+            return [None] * n, [None] * n
+        if isinstance(b, (list, tuple)) and len(b) == n and (np.ndim(b[0]) == 1 or hasattr(b[0], "edges")):
+            return [as_edges(e) for e in b], [rule_of(e) for e in b]
         e = as_edges(b)
-        return [e] * n
+        return [e] * n, [rule_of(b)] * n
 
-    des, ves = per_slice(distance_bins), per_slice(value_bins)
+    (des, drs), (ves, vrs) = per_slice(distance_bins), per_slice(value_bins)
     slices = []
     for x, u in zip(xs, us):
         x = np.asarray(x, dtype=np.float64)
@@ -296,7 +317,7 @@ def calculate_structure_function_batch(sf, xs, us, distance_bins, value_bins=Non
         slices.append((x, u))
     if n == 0:
         return []
-    sums, counts, nd, nv = _run(sf, slices, des, ves, distance_metric, backend)
+    sums, counts, nd, nv = _run(sf, slices, des, ves, distance_metric, backend, _native.SEM_POINT, drs, vrs)
     ncell = nd * max(nv, 1)
     out = []
     for b in range(n):

@@ -26,7 +26,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import _native
-from .bins import as_edges
+from .bins import as_edges, rule_of
 from .dispatch import (MI355XBackend, _assert_counts_representable, _finalize, _run, _semantics, _slices,
                        _validate_array_shape)
 from .errors import DimensionMismatch
@@ -51,7 +51,7 @@ def _single_pass_raw(x, u, distance_bins, backend, distance_metric):
     de = as_edges(distance_bins)
     slices, aux = _slices(x, u, kind)
     sums, counts, nd, _ = _run(_SinglePassOp(), slices, [de] * len(slices), [None] * len(slices),
-                               distance_metric, backend, _semantics(kind))
+                               distance_metric, backend, _semantics(kind), [rule_of(distance_bins)] * len(slices))
     B = len(slices)
     s = sums.reshape(B, SINGLE_PASS_N * nd).T.reshape((SINGLE_PASS_N, nd) + aux, order="F")
     c = counts.reshape(B, SINGLE_PASS_N * nd).T.reshape((SINGLE_PASS_N, nd) + aux, order="F")
@@ -132,7 +132,7 @@ def _single_pass_2d_raw(x, u, distance_bins, vlist, n_val, backend, distance_met
         [np.asarray(as_edges(v), dtype=np.float64)[:n_val + 1] for v in vlist]))
     slices, aux = _slices(x, u, kind)
     sums, counts, nd, nv = _run(_SinglePassOp(), slices, [de] * len(slices), [ve6] * len(slices),
-                                distance_metric, backend, _semantics(kind))
+                                distance_metric, backend, _semantics(kind), [rule_of(distance_bins)] * len(slices))
     B = len(slices)
     ncell = SINGLE_PASS_N * nd * nv
     s = sums.reshape(B, ncell).T.reshape((SINGLE_PASS_N, nd, nv) + aux, order="F")

@@ -14,6 +14,13 @@ PATH = os.path.join(ROOT, "oracle", "libsforacle.so")
 GEOM = {"flat": 0, "deg": 1, "rad": 2}
 
 
+class BinRule(ctypes.Structure):
+    """sfo_bin_rule: the fields of a LinearBinEdges (src/BinEdges.jl:94-100), optionally padded."""
+    _fields_ = [("kind", ctypes.c_int32), ("padded", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("first_edge", ctypes.c_double), ("last_edge", ctypes.c_double),
+                ("inv_step", ctypes.c_double), ("step_val", ctypes.c_double)]
+
+
 class Prob(ctypes.Structure):
     _fields_ = [
         ("D", ctypes.c_int32), ("W", ctypes.c_int32), ("N", ctypes.c_int64),
@@ -23,6 +30,7 @@ class Prob(ctypes.Structure):
         ("n_dist", ctypes.c_int32), ("dist_edges", ctypes.c_void_p),
         ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p),
         ("batch_leading", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("dist_rule", ctypes.c_void_p), ("val_rule", ctypes.c_void_p),
     ]
 
 
@@ -46,6 +54,8 @@ def lib():
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]
         L.sfo_hist_single_pass_2d.restype = ctypes.c_int
+        L.sfo_rule_digitize.argtypes = [ctypes.POINTER(BinRule), ctypes.c_double]
+        L.sfo_rule_digitize.restype = ctypes.c_int64
         L.sfo_atan2.argtypes = [ctypes.c_double, ctypes.c_double]
         L.sfo_atan2.restype = ctypes.c_double
         L.sfo_sincosd.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -58,14 +68,33 @@ def lib():
     return _lib
 
 
+def _rule(fields):
+    if fields is None:
+        return None
+    return BinRule(1, 1 if fields["padded"] else 0, int(fields["n"]), float(fields["first_edge"]),
+                   float(fields["last_edge"]), float(fields["inv_step"]), float(fields["step_val"]))
+
+
+def rule_digitize(fields, x):
+    """digitize(x, axis) of a LinearBinEdges axis (dict of rule fields, sfamd.bins.rule_of)."""
+    r = _rule(fields)
+    return int(lib().sfo_rule_digitize(ctypes.byref(r), float(x)))
+
+
 def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, batch_leading=False):
     """batch_leading: evaluate flat pairs as the reference's batch-leading kernels (fields with
-    auxiliary axes) rather than its point-field kernels."""
+    auxiliary axes) rather than its point-field kernels.  edges / vedges may be sfamd bin
+    specifications: a LinearBinEdges axis is digitized by its rule."""
+    from sfamd.bins import as_edges, rule_of
     keep = []
     xc = np.asfortranarray(x, dtype=np.float64)
     uc = np.asfortranarray(u, dtype=np.float64)
-    de = np.ascontiguousarray(edges, dtype=np.float64)
-    keep += [xc, uc, de]
+    drule = _rule(rule_of(edges))
+    vrule = None if vedges is None else _rule(rule_of(vedges))
+    de = np.ascontiguousarray(as_edges(edges), dtype=np.float64)
+    if vedges is not None:
+        vedges = as_edges(vedges)
+    keep += [xc, uc, de, drule, vrule]
     p = Prob()
     p.D, p.W, p.N = uc.shape[0], xc.shape[0], xc.shape[1]
     p.x, p.u = xc.ctypes.data, uc.ctypes.data
@@ -80,6 +109,10 @@ def make(x, u, edges, op=(0, 2, 0), geometry="flat", radius=1.0, vedges=None, ba
         keep.append(ve)
         p.n_val, p.val_edges = len(ve) - 1, ve.ctypes.data
     p.batch_leading = 1 if batch_leading else 0
+    if drule is not None:
+        p.dist_rule = ctypes.addressof(drule)
+    if vrule is not None:
+        p.val_rule = ctypes.addressof(vrule)
     return p, keep
 
 
