@@ -174,6 +174,22 @@ int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d
 int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
                          uint64_t *d_counts, void *stream);
 
+/* Allocation-free variant of sf_hist_batch_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the call makes no hipMalloc or
+ * hipFree: every device buffer of the call (records, bin tables, work items, accumulators, the
+ * re-bin queue) is carved from one static device array of SF_NOALLOC_SCRATCH_BYTES per device,
+ * held until the process ends.  Limits: the records of all slices must fit in half of it (about
+ * 400 000 points for the sphere in a binary32 precision, 700 000 in binary64) and the rest of the
+ * call in the remainder, else SF_E_UNSUPPORTED is returned; the re-bin queue gets whatever is left,
+ * and a call that overflows it is redone with the binary64 kernel (slower, same results); no tile
+ * culling.  Every call on a device shares the array and leaves its contents behind (what must
+ * start at zero is cleared by each call): calls that overlap in time on one device race, so
+ * serialize them (one stream per device). */
+#define SF_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int sf_hist_batch_device_noalloc(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices,
+                                 double *d_sums, uint64_t *d_counts, void *stream);
+
 /* Autotuner: times candidate launch shapes on these slices (device pointers, as for
  * sf_hist_batch_device; outputs are discarded) -- threads per workgroup, column-record variant,
  * the histogram's privatisation (1D: 4..32 lane-group copies; joint: 1 or 2 copies by lane halves)

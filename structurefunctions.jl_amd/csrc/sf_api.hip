@@ -57,13 +57,38 @@ struct PrepSliceHost {
 
 using namespace sfk;
 
+// Device scratch of the allocation-free entry (sf_hist_batch_device_noalloc): records, bin tables,
+// work items, accumulators and the re-bin queue of one call, carved in that order.  One copy per
+// device, held until the process ends; every call on a device shares it.
+static __device__ __align__(256) char scratch[SF_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+// A bump allocator over a caller-provided device region (the allocation-free entry's scratch).
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+};
 
 template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t cap = 0;  // elements
+  bool borrowed = false;  // p points into an Arena: never freed
+  // the next n elements of the arena (256-byte aligned); hipErrorOutOfMemory when they do not fit
+  hipError_t take(size_t n, Arena &a) {
+    if (p && !borrowed) return hipErrorInvalidValue;  // would leak an owned allocation
+    const size_t want = std::max<size_t>(n, 1);
+    const size_t off = (a.used + 255) & ~(size_t)255;
+    if (off > a.cap || want > (a.cap - off) / sizeof(T)) return hipErrorOutOfMemory;
+    p = (T *)(a.base + off);
+    cap = want;
+    borrowed = true;
+    a.used = off + want * sizeof(T);
+    return hipSuccess;
+  }
   hipError_t ensure(size_t n) {
+    if (borrowed) return hipErrorInvalidValue;
     if (n <= cap && p) return hipSuccess;
     if (p) {
       hipError_t e = hipFree(p);
@@ -77,9 +102,10 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     cap = 0;
+    borrowed = false;
   }
 };
 
@@ -192,6 +218,7 @@ struct Plan {
   DevBuf<int> flag;
   DevBuf<int32_t> perm;   // spatial order of the binary32 records (tile culling)
   bool culled = false;    // some work items were dropped by tile culling
+  Arena *arena = nullptr;  // set: every buffer is taken from this region (no device allocation)
   hipGraphExec_t graph = nullptr;
   double *g_sums = nullptr;
   unsigned long long *g_counts = nullptr;
@@ -260,6 +287,10 @@ struct sf_ctx {
   // device buffers: host-input staging and the plan reused by one-shot calls
   DevBuf<double> d_x, d_u;
   Plan oneshot;
+  // the allocation-free entry: its plan (buffers borrowed from the scratch array) and this
+  // device's address of the array
+  Plan noalloc;
+  char *scratch_ptr = nullptr;
   // spatial sort and tile culling: device workspace and the slice/tile boxes read back
   DevBuf<char> cull_ws;
   DevBuf<double> cull_info;
@@ -652,6 +683,12 @@ extern "C" int sf_create(int device, sf_ctx **out) {
   if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
     ctx->max_smem = (size_t)v;
   if (hipStreamCreate(&ctx->stream) != hipSuccess) { delete ctx; return SF_E_DEVICE; }
+  // this device's copy of the allocation-free entry's scratch array (the array's name is a host
+  // stand-in; only this address is used on the host)
+  if (hipGetSymbolAddress((void **)&ctx->scratch_ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+    ctx->scratch_ptr = nullptr;
+    (void)hipGetLastError();
+  }
   *out = ctx;
   return SF_OK;
 }
@@ -663,6 +700,7 @@ extern "C" void sf_destroy(sf_ctx *ctx) {
   ctx->d_x.release();
   ctx->d_u.release();
   ctx->oneshot.release();
+  ctx->noalloc.release();
   ctx->cull_ws.release();
   ctx->cull_info.release();
   for (int k = 0; k < 4; ++k)
@@ -845,6 +883,17 @@ int64_t interleave_mul(int64_t N) {
   return m;
 }
 
+// a plan buffer: allocated, or (allocation-free entry) taken from the plan's arena
+#define PLBUF(buf, n)                                                                             \
+  do {                                                                                            \
+    const hipError_t e_ = pl.arena ? (buf).take((size_t)(n), *pl.arena) : (buf).ensure((size_t)(n)); \
+    if (e_ == hipErrorOutOfMemory && pl.arena)                                                    \
+      return ctx->fail(SF_E_UNSUPPORTED,                                                          \
+                       "this call needs more device scratch than the allocation-free entry's %llu MiB " \
+                       "(at %s); use sf_hist_batch_device", (unsigned long long)(SF_NOALLOC_SCRATCH_BYTES >> 20), #buf); \
+    if (e_ != hipSuccess) return ctx->fail(SF_E_DEVICE, "%s: %s", #buf, hipGetErrorString(e_));   \
+  } while (0)
+
 // Host work and uploads for one batch.  x/u of each slice are device pointers.
 int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const double *const *dx_in,
                const double *const *du_in, const double *xmax_in, hipStream_t st, bool cull) {
@@ -925,9 +974,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     pl.max_n = std::max<int64_t>(pl.max_n, sl[s].N);
     pl.total_pairs += pairs_of(sl[s].N);
   }
-  HIPCHK(pl.rec64.ensure((size_t)pl.total_pts * S64 + 16));
+  PLBUF(pl.rec64, ((size_t)pl.total_pts * S64 + 16));
   // + one record (F32 <= 16): the binary32 pair loop requests the record after its last column
-  if (mixed) HIPCHK(pl.rec32.ensure((size_t)pl.total_pts * F32 + 16));
+  if (mixed) PLBUF(pl.rec32, ((size_t)pl.total_pts * F32 + 16));
 
   // ---- tables
   std::vector<Slice> sdesc(n);
@@ -1067,14 +1116,14 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   pl.culled = false;
   if (cull && mixed && !strip) {
     // the record order of slice s lands at pl.perm + pt_off (used only where culling is active)
-    HIPCHK(pl.perm.ensure((size_t)pl.total_pts));
+    PLBUF(pl.perm, ((size_t)pl.total_pts));
     for (int s = 0; s < n; ++s) {
       const int rc = cull_prepare(ctx, sl[s], dx[s], xmax[s], block, st, cinfo[s], pl.perm.p + prep[s].pt_off);
       if (rc) return rc;
       if (cinfo[s].active) prep[s].perm = pl.perm.p + prep[s].pt_off;
     }
   }
-  HIPCHK(pl.prep.ensure(n));
+  PLBUF(pl.prep, (n));
   HIPCHK(hipMemcpyAsync(pl.prep.p, prep.data(), sizeof(PrepSliceHost) * n, hipMemcpyHostToDevice, st));
 
   std::vector<int> order(n);
@@ -1134,21 +1183,21 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   pl.sload = sload;
 
   // ---- upload
-  HIPCHK(pl.slices.ensure(n));
-  HIPCHK(pl.items.ensure(items.size()));
-  HIPCHK(pl.cells.ensure(cells.size()));
-  HIPCHK(pl.dedges.ensure(dedges.size()));
-  HIPCHK(pl.vedges.ensure(vedges.size()));
-  HIPCHK(pl.vedges32.ensure(vedges32.size()));
-  HIPCHK(pl.acc_s.ensure((size_t)pl.nout * ncell));
-  HIPCHK(pl.acc_c.ensure((size_t)pl.nout * ncell));
-  HIPCHK(pl.counter.ensure(4));
-  HIPCHK(pl.pcount.ensure(4));
-  HIPCHK(pl.flag.ensure(4));
+  PLBUF(pl.slices, (n));
+  PLBUF(pl.items, (items.size()));
+  PLBUF(pl.cells, (cells.size()));
+  PLBUF(pl.dedges, (dedges.size()));
+  PLBUF(pl.vedges, (vedges.size()));
+  PLBUF(pl.vedges32, (vedges32.size()));
+  PLBUF(pl.acc_s, ((size_t)pl.nout * ncell));
+  PLBUF(pl.acc_c, ((size_t)pl.nout * ncell));
+  PLBUF(pl.counter, (4));
+  PLBUF(pl.pcount, (4));
+  PLBUF(pl.flag, (4));
   HIPCHK(hipMemcpyAsync(pl.slices.p, sdesc.data(), sizeof(Slice) * n, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(pl.items.p, items.data(), sizeof(Item) * items.size(), hipMemcpyHostToDevice, st));
   if (strip) {
-    HIPCHK(pl.items64.ensure(items64.size()));
+    PLBUF(pl.items64, (items64.size()));
     HIPCHK(hipMemcpyAsync(pl.items64.p, items64.data(), sizeof(Item) * items64.size(), hipMemcpyHostToDevice, st));
   }
   HIPCHK(hipMemcpyAsync(pl.cells.p, cells.data(), sizeof(Cell) * cells.size(), hipMemcpyHostToDevice, st));
@@ -1163,7 +1212,17 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     pl.cap = std::max<int64_t>(1LL << 22, pl.total_pairs / 2000);
     pl.cap = std::min<int64_t>(pl.cap, 1LL << 27);
     pl.cap = std::max<int64_t>(pl.cap, std::min<int64_t>(ctx->queue_hint, kQueueMax));
-    HIPCHK(pl.pending.ensure((size_t)pl.cap));
+    if (pl.arena) {
+      // the re-bin queue gets the rest of the scratch (an overflow is redone in binary64)
+      const size_t off = (pl.arena->used + 255) & ~(size_t)255;
+      const int64_t room = off < pl.arena->cap ? (int64_t)((pl.arena->cap - off) / sizeof(Pending)) : 0;
+      pl.cap = std::min<int64_t>(pl.cap, room);
+      if (pl.cap < 1024)
+        return ctx->fail(SF_E_UNSUPPORTED,
+                         "this call leaves no room for the re-bin queue in the allocation-free entry's "
+                         "%llu MiB scratch; use sf_hist_batch_device", (unsigned long long)(SF_NOALLOC_SCRATCH_BYTES >> 20));
+    }
+    PLBUF(pl.pending, ((size_t)pl.cap));
   }
 
   Params &P = pl.P;
@@ -1415,6 +1474,73 @@ extern "C" int sf_hist_batch_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
 extern "C" int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d_counts,
                               void *stream) {
   return sf_hist_batch_device(ctx, 1, p, d_sums, d_counts, stream);
+}
+
+// ----------------------------------------------------------------------------------------------
+// allocation-free entry: every device buffer of the call comes from one static array per device
+// ----------------------------------------------------------------------------------------------
+extern "C" int sf_hist_batch_device_noalloc(sf_ctx *ctx, int32_t n, const sf_problem *sl, double *d_sums,
+                                            uint64_t *d_counts, void *stream) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  int rc = validate_batch(ctx, n, sl, false);
+  if (rc) return rc;
+  if (n == 0) return SF_OK;
+  if (!d_sums || !d_counts) return ctx->fail(SF_E_ARG, "null sums or counts");
+  if (!ctx->scratch_ptr) return ctx->fail(SF_E_DEVICE, "the scratch array has no address on this device");
+  // the records alone must fit (checked before any device work; the tables, items, accumulators
+  // and queue are checked as the plan takes them)
+  int64_t pts = 0;
+  for (int s = 0; s < n; ++s) pts += sl[s].N;
+  const int G = sl[0].geometry == SF_GEOM_FLAT ? kFlat : kSphere;
+  const int64_t rec_bytes = pts * (8 * rec_d64(G, sl[0].D) + (sl[0].precision != SF_PREC_FP64 ? 4 * rec_f32(G, sl[0].D) : 0));
+  if (rec_bytes > (int64_t)(SF_NOALLOC_SCRATCH_BYTES / 2))
+    return ctx->fail(SF_E_UNSUPPORTED,
+                     "%lld points need %lld bytes of records, more than half of the allocation-free entry's "
+                     "%llu MiB scratch; use sf_hist_batch_device", (long long)pts, (long long)rec_bytes,
+                     (unsigned long long)(SF_NOALLOC_SCRATCH_BYTES >> 20));
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+    if (sl[s].precision != SF_PREC_FP64 && sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+      std::vector<double> hx((size_t)sl[s].W * sl[s].N);
+      HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double m = 0;
+      for (double v : hx) m = std::max(m, fabs(v));
+      xmax[s] = m;
+    }
+  }
+  Plan &pl = ctx->noalloc;
+  Arena arena{ctx->scratch_ptr, (size_t)SF_NOALLOC_SCRATCH_BYTES, 0};
+  pl.arena = &arena;
+  // no culling: its sort workspace would be another buffer
+  rc = plan_build(ctx, pl, n, sl, px.data(), pu.data(), xmax.data(), st, false);
+  pl.arena = nullptr;
+  if (rc) return rc;
+  for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
+  ctx->stats[0] = pl.total_pairs;
+  ctx->stats[2] = pl.n_items;
+  ctx->stats[3] = pl.grid;
+  ctx->stats[7] = pl.block;
+  ctx->ev_pending = false;
+  if (pl.total_pairs == 0) return SF_OK;
+  unsigned long long *dc = (unsigned long long *)d_counts;
+  rc = plan_launch(pl, pl.mixed, d_sums, dc, st, nullptr);
+  if (rc) return rc;
+  if (pl.mixed) {
+    unsigned long long pc = 0;
+    HIPCHK(hipMemcpyAsync(&pc, pl.pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats[1] = (int64_t)pc;
+    // the queue overflowed its share of the scratch: the merge was skipped, redo in binary64
+    if ((int64_t)pc > pl.cap) rc = plan_launch(pl, false, d_sums, dc, st, nullptr);
+  }
+  return rc;
 }
 
 // ----------------------------------------------------------------------------------------------

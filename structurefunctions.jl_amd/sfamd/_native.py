@@ -25,7 +25,7 @@ EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batc
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_device_info", "sf_check",
             "sf_eval_math",
             "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy",
-            "sf_autotune")
+            "sf_autotune", "sf_hist_batch_device_noalloc")
 
 
 class SfBinRule(ctypes.Structure):
@@ -126,6 +126,8 @@ def lib():
     L.sf_plan_destroy.restype = None
     L.sf_autotune.argtypes = [vp, i32, ctypes.POINTER(SfProblem), ctypes.POINTER(i32)]
     L.sf_autotune.restype = ctypes.c_int
+    L.sf_hist_batch_device_noalloc.argtypes = [vp, i32, ctypes.POINTER(SfProblem), dp, dp, vp]
+    L.sf_hist_batch_device_noalloc.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -213,6 +215,14 @@ class Context:
         """Host-pointer batch call; sums/counts are contiguous arrays (n_slices * ncell)."""
         arr = (SfProblem * len(problems))(*problems)
         rc = lib().sf_hist_batch(self._h, len(problems), arr, sums.ctypes.data, counts.ctypes.data)
+        self.check(rc)
+
+    def hist_device_noalloc(self, problems, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0):
+        """sf_hist_batch_device_noalloc: as hist_device, with no device allocation."""
+        arr = (SfProblem * len(problems))(*problems)
+        rc = lib().sf_hist_batch_device_noalloc(self._h, len(problems), arr, ctypes.c_void_p(d_sums_ptr),
+                                                ctypes.c_void_p(d_counts_ptr),
+                                                ctypes.c_void_p(stream_ptr) if stream_ptr else None)
         self.check(rc)
 
     def hist_device(self, problems, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0):
