@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64 comparison step")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="skip the timing of every output of the configuration in one fused call")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device copy of 1 GiB (counter calibration for HBM traffic)")
@@ -211,6 +213,9 @@ def main():
         if args.precision == "mixed_fixed":
             extra["mixed_gpairs_s"] = round(rate("mixed"), 3)
 
+    if not args.no_fused and args.precision != "fp64":
+        extra["all_outputs"] = fused_vs_separate(ctx, stream, dx, du, N, dedges, vedges, args.precision)
+
     if args.copy_probe:
         a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -266,6 +271,63 @@ def main():
     }
     out.update(extra)
     print(json.dumps(out))
+
+
+def fused_vs_separate(ctx, stream, dx, du, N, dedges, vedges, prec):
+    """Every output the spherical configuration asks for -- du_L^2, du_T^2, du_L^3, du_T^3 over
+    the 64 distance bins and the joint du_L histogram -- as one fused call (one evaluation of each
+    pair) and as five separate calls; wall time of each (best of three), counts compared."""
+    import numpy as np
+    import torch
+
+    from sfamd import _native
+
+    nd, nv = len(dedges) - 1, len(vedges) - 1
+    names = ["L2", "T2", "L3", "T3", "joint L1"]
+    dev = dx.device
+    bufs = [(torch.zeros(nd * (nv if n == "joint L1" else 1), dtype=torch.float64, device=dev),
+             torch.zeros(nd * (nv if n == "joint L1" else 1), dtype=torch.int64, device=dev)) for n in names]
+    outs = []
+    for n in names:
+        o = _native.SfOutput()
+        o.op, o.nl, o.nt = OPS[n]
+        if n == "joint L1":
+            o.n_val, o.val_edges = nv, vedges.ctypes.data
+        outs.append(o)
+    p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, 2, 2, 1, OPS["L2"], dedges, None, prec)
+
+    def run_fused():
+        ctx.hist_multi_device([p], outs, [b[0].data_ptr() for b in bufs], [b[1].data_ptr() for b in bufs], stream)
+
+    def run_separate():
+        for n, (bs, bc) in zip(names, bufs):
+            q = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, 2, 2, 1, OPS[n], dedges,
+                                vedges if n == "joint L1" else None, prec)
+            ctx.hist_device([q], bs.data_ptr(), bc.data_ptr(), stream)
+
+    res = {}
+    counts = {}
+    for label, fn in (("fused", run_fused), ("separate", run_separate)):
+        best = None
+        for r in range(4):
+            for bs, bc in bufs:
+                bs.zero_()
+                bc.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if r:
+                best = dt if best is None else min(best, dt)
+        res[f"{label}_ms"] = round(best * 1e3, 3)
+        counts[label] = [bc.cpu().numpy().copy() for _, bc in bufs]
+    pairs = N * (N - 1) // 2
+    res["outputs"] = names
+    res["fused_gpairs_s"] = round(pairs / res["fused_ms"] / 1e6, 3)
+    res["speedup_fused_vs_separate"] = round(res["separate_ms"] / res["fused_ms"], 3)
+    res["counts_equal"] = all(np.array_equal(a, b) for a, b in zip(counts["fused"], counts["separate"]))
+    return res
 
 
 def host_info():

@@ -174,6 +174,34 @@ int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d
 int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, double *d_sums,
                          uint64_t *d_counts, void *stream);
 
+/* Fused outputs: one evaluation of every pair feeds several operators of the same point fields.
+ * The slices give the fields, geometry, distance bins, precision and semantics as for
+ * sf_hist_batch_device (their op, nl, nt, n_val, val_edges and val_rule are ignored); outs[k]
+ * gives operator k and, for at most one of them, value bins (a joint histogram).  Output k of
+ * slice s accumulates into d_sums[k] / d_counts[k] at s * ncell_k (ncell_k = n_dist, or
+ * n_dist * n_val for the joint output).  Results equal n_out separate calls (counts bit-exact;
+ * sums to each precision's bound); the binary32 precisions evaluate the pair geometry, the frame
+ * and the increments once for all outputs, SF_PREC_FP64 runs the outputs one after the other.
+ * At most SF_MAX_FUSED 1D outputs plus one joint output; SF_OP_SINGLE_PASS is not an output (it
+ * is its own fused form).  No reference counterpart as one call: the reference evaluates several
+ * moments per pair in its GPU histogram kernel (ext/gpu/sf_tiled.jl:7-13, 128-131) and shares a
+ * pair's geometry across fields in its batch-leading loop (src/Calculations/batch_leading.jl:105-140);
+ * each output is calculate_structure_function(outs[k] operator, x, u, distance_bins[, value_bins]). */
+#define SF_MAX_FUSED 8
+typedef struct {
+  int32_t op, nl, nt;            /* operator (SF_OP_*, exponents)                                 */
+  int32_t n_val;                 /* 0: 1D over the distance bins; else the joint histogram's value
+                                    bins (n_val + 1 ascending edges in val_edges, host pointer)    */
+  const double *val_edges;
+  const sf_bin_rule *val_rule;   /* NULL or the value axis' LinearBinEdges rule                   */
+} sf_output;
+int sf_hist_multi_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t n_out,
+                         const sf_output *outs, double *const *d_sums, uint64_t *const *d_counts,
+                         void *stream);
+/* Host-pointer form of sf_hist_multi_device (x, u, sums and counts in host memory; synchronous). */
+int sf_hist_multi(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t n_out,
+                  const sf_output *outs, double *const *sums, uint64_t *const *counts);
+
 /* Allocation-free variant of sf_hist_batch_device.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments and results; the call makes no hipMalloc or

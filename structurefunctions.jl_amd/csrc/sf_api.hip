@@ -219,6 +219,13 @@ struct Plan {
   DevBuf<int32_t> perm;   // spatial order of the binary32 records (tile culling)
   bool culled = false;    // some work items were dropped by tile culling
   Arena *arena = nullptr;  // set: every buffer is taken from this region (no device allocation)
+  // fused outputs: 1D operators evaluated with the main (joint) one, their accumulators and the
+  // caller's outputs they merge into
+  int n1d = 0, mcopies = 0;
+  DevBuf<double> msums;
+  DevBuf<unsigned long long> mcounts;
+  double *mdst_s[kMaxFused] = {};
+  unsigned long long *mdst_c[kMaxFused] = {};
   hipGraphExec_t graph = nullptr;
   double *g_sums = nullptr;
   unsigned long long *g_counts = nullptr;
@@ -231,6 +238,8 @@ struct Plan {
     counter.release(); pcount.release(); pending.release(); prep.release(); flag.release();
     items64.release();
     perm.release();
+    msums.release();
+    mcounts.release();
   }
 };
 
@@ -291,6 +300,7 @@ struct sf_ctx {
   // device's address of the array
   Plan noalloc;
   char *scratch_ptr = nullptr;
+  Plan fused;  // the fused-output entry's plan
   // spatial sort and tile culling: device workspace and the slice/tile boxes read back
   DevBuf<char> cull_ws;
   DevBuf<double> cull_info;
@@ -587,7 +597,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   s.v_uniform = 0;
   s.v_negzero = 0;
   s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0; s.vabs = 0;
-  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0;
+  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0; s.vsmall = 0; s.gz = 0;
   s.v_log = 0; s.vk0 = 0; s.vl0f = 0; s.vlinvf = 0;
   if (p.n_val > 0) {
     const int nv = p.n_val;
@@ -614,6 +624,11 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
           s.v_fx = 1;
           s.vwf = (float)w;
           s.vsf = kFxScale / s.vwf;
+          // values with |v| < w/4 lie in the value bins floor((-w/4 - v0)/w) and the next; one
+          // more bin below covers rounding
+          s.vsmall = (float)(0.25L * w);
+          const long double gzl = floorl((-0.25L * w - (long double)v0) / w) - 1.0L;
+          s.gz = (int32_t)std::max(-1.0e9L, std::min(1.0e9L, gzl));
         }
       }
     }
@@ -701,6 +716,7 @@ extern "C" void sf_destroy(sf_ctx *ctx) {
   ctx->d_u.release();
   ctx->oneshot.release();
   ctx->noalloc.release();
+  ctx->fused.release();
   ctx->cull_ws.release();
   ctx->cull_info.release();
   for (int k = 0; k < 4; ++k)
@@ -883,6 +899,12 @@ int64_t interleave_mul(int64_t N) {
   return m;
 }
 
+// Fused outputs: the 1D operators a fused plan evaluates besides its main operator.
+struct MultiSpec {
+  int n1d = 0;
+  int op[kMaxFused] = {}, nl[kMaxFused] = {}, nt[kMaxFused] = {};
+};
+
 // a plan buffer: allocated, or (allocation-free entry) taken from the plan's arena
 #define PLBUF(buf, n)                                                                             \
   do {                                                                                            \
@@ -896,7 +918,8 @@ int64_t interleave_mul(int64_t N) {
 
 // Host work and uploads for one batch.  x/u of each slice are device pointers.
 int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const double *const *dx_in,
-               const double *const *du_in, const double *xmax_in, hipStream_t st, bool cull) {
+               const double *const *du_in, const double *xmax_in, hipStream_t st, bool cull,
+               const MultiSpec *multi = nullptr) {
   // the single pass's 2D form runs every input slice as six plane slices (one joint histogram per
   // invariant, its own value edges); plane t of input slice s writes output cells t, t + 6, ...
   const bool planes = sp2d(sl_in[0].op, sl_in[0].n_val);
@@ -937,6 +960,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   pl.mixed = p0.precision != SF_PREC_FP64;
   pl.ncell = out_cells(p0);  // per input slice
   pl.sp = p0.op == SF_OP_SINGLE_PASS && !planes;
+  pl.n1d = multi ? multi->n1d : 0;
   // B-strip: slices that share positions (the auxiliary axes of one field: same x, N, edges) and
   // bin 1D histograms in binary32 evaluate each pair's geometry once for up to kStripMax fields.
   // Opt-in (tuning variant 3): the per-slice 1D kernel with 16 histogram copies outruns it (eight
@@ -1026,6 +1050,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   Pk.nt = p0.nt;
   Pk.n_val = p0.n_val;
   Pk.nmom = pl.sp ? 4 : 1;
+  Pk.n1d = pl.n1d;
   const int nmom = Pk.nmom;
   Tuning tn = ctx->tune;
   if (!tn.any()) {
@@ -1065,8 +1090,18 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     copies = 2;  // joint histogram privatised by lane halves (binary32 kernel only)
   }
   const int jlayout = (J && tn.layout) ? tn.layout : 1;
+  // fused 1D outputs: lane-group copies of their histograms (fewer beside a joint histogram, so
+  // that the LDS still holds two workgroups per CU)
+  int mcop = 0;
+  if (pl.n1d) {
+    mcop = J ? 4 : 16;
+    while (mcop > 1 && (size_t)smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0,
+                                           strip, jlayout, pl.n1d, mcop).total > ctx->max_smem)
+      mcop >>= 1;
+  }
+  pl.mcopies = mcop;
   const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0, strip,
-                                    jlayout);
+                                    jlayout, pl.n1d, mcop);
   // the binary64 kernel keeps one copy of a joint histogram
   const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, J ? 1 : copies, nmom, 0, 0, 0,
                                      jlayout);
@@ -1091,7 +1126,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   if (mixed) {
     for (int q = kQueueCap; q >= 64 && qcap == 0; q >>= 1) {
       const int t = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, q, strip,
-                                jlayout).total;
+                                jlayout, pl.n1d, mcop).total;
       if ((size_t)t <= ctx->max_smem && occupancy(block, (size_t)t) >= bpc) qcap = q;
     }
   }
@@ -1191,6 +1226,10 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   PLBUF(pl.vedges32, (vedges32.size()));
   PLBUF(pl.acc_s, ((size_t)pl.nout * ncell));
   PLBUF(pl.acc_c, ((size_t)pl.nout * ncell));
+  if (pl.n1d) {
+    PLBUF(pl.msums, ((size_t)pl.n1d * pl.nout * p0.n_dist));
+    PLBUF(pl.mcounts, ((size_t)pl.n1d * pl.nout * p0.n_dist));
+  }
   PLBUF(pl.counter, (4));
   PLBUF(pl.pcount, (4));
   PLBUF(pl.flag, (4));
@@ -1264,6 +1303,16 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.qw = qcap / (block / 64);
   P.bl = (flat && p0.semantics == SF_SEM_BATCH) ? 1 : 0;
   P.strip = strip;
+  P.n1d = pl.n1d;
+  P.mcopies = mcop;
+  P.n_slices = pl.nout;
+  for (int k = 0; k < pl.n1d; ++k) {
+    P.mop[k] = multi->op[k];
+    P.mnl[k] = multi->nl[k];
+    P.mnt[k] = multi->nt[k];
+  }
+  P.msums = pl.n1d ? pl.msums.p : nullptr;
+  P.mcounts = pl.n1d ? pl.mcounts.p : nullptr;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
   return SF_OK;
 }
@@ -1283,6 +1332,13 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   HIPCHK(launch_zero(pl.counter.p, sizeof(int) * 4, st));
   HIPCHK(launch_zero(pl.pcount.p, sizeof(unsigned long long) * 4, st));
   HIPCHK(launch_zero(pl.flag.p, sizeof(int) * 4, st));
+  const int64_t mcells = (int64_t)pl.n1d * pl.nout * pl.P.n_dist;
+  if (pl.n1d) {
+    if (!use_mixed)  // fused outputs exist in the binary32 kernel only
+      return ctx->fail(SF_E_UNSUPPORTED, "internal: fused outputs launched on the binary64 kernel");
+    HIPCHK(launch_zero(pl.msums.p, sizeof(double) * mcells, st));
+    HIPCHK(launch_zero(pl.mcounts.p, sizeof(unsigned long long) * mcells, st));
+  }
   const bool use_sload = use_mixed && pl.sload;
   const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
   Params P = pl.P;
@@ -1302,7 +1358,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int stage_mixed = stage * ((use_mixed && pl.strip) ? pl.strip : 1);
   P.stage_bytes = stage_mixed;
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_mixed, P.copies, P.nmom, P.fx, P.qcap,
-                                   P.strip, P.jlayout);
+                                   P.strip, P.jlayout, P.n1d, P.mcopies);
   if (ev) HIPCHK(hipEventRecord(ev[0], st));
   HIPCHK(launch_hist(pl.G, pl.D, pl.J, use_mixed, use_sload, P, pl.grid, pl.block, (size_t)L.total, st));
   if (ev) HIPCHK(hipEventRecord(ev[1], st));
@@ -1310,8 +1366,14 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   // 2.7e8 queued pairs: 22.6 / 16.7 / 19.6 ms with one / four / eight per CU)
   if (use_mixed) HIPCHK(launch_recheck(pl.G, pl.D, pl.J, P, 4 * ctx->n_cu, st));
   if (ev) HIPCHK(hipEventRecord(ev[2], st));
-  HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
-                              use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
+  // (a fused call without a joint output passes no out_s / out_c)
+  if (out_s)
+    HIPCHK(launch_merge_checked(pl.acc_s.p, pl.acc_c.p, out_s, out_c, ncells,
+                                use_mixed ? pl.pcount.p : nullptr, pl.cap, pl.flag.p, st));
+  const int64_t per = (int64_t)pl.nout * pl.P.n_dist;
+  for (int k = 0; k < pl.n1d; ++k)
+    HIPCHK(launch_merge_checked(pl.msums.p + k * per, pl.mcounts.p + k * per, pl.mdst_s[k], pl.mdst_c[k], per,
+                                pl.pcount.p, pl.cap, pl.flag.p, st));
   if (pl.sp) HIPCHK(launch_derive_sp(out_s, out_c, (int64_t)pl.nout * pl.P.n_dist, st));
   if (ev) HIPCHK(hipEventRecord(ev[3], st));
   return SF_OK;
@@ -1474,6 +1536,206 @@ extern "C" int sf_hist_batch_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
 extern "C" int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, uint64_t *d_counts,
                               void *stream) {
   return sf_hist_batch_device(ctx, 1, p, d_sums, d_counts, stream);
+}
+
+// ----------------------------------------------------------------------------------------------
+// fused outputs: one pair evaluation, several operators
+// ----------------------------------------------------------------------------------------------
+extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out,
+                                    const sf_output *outs, double *const *d_sums, uint64_t *const *d_counts,
+                                    void *stream) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  if (n_out < 1 || !outs || !d_sums || !d_counts) return ctx->fail(SF_E_ARG, "a fused call needs at least one output");
+  int joint = -1, n1 = 0, first1 = -1;
+  for (int k = 0; k < n_out; ++k) {
+    if (!d_sums[k] || !d_counts[k]) return ctx->fail(SF_E_ARG, "output %d: null sums or counts", k);
+    if (outs[k].op == SF_OP_SINGLE_PASS)
+      return ctx->fail(SF_E_ARG, "output %d: the single pass is not a fused output (it is its own fused form)", k);
+    if (outs[k].n_val > 0) {
+      if (joint >= 0) return ctx->fail(SF_E_UNSUPPORTED, "a fused call takes at most one joint output");
+      joint = k;
+    } else {
+      if (first1 < 0) first1 = k;
+      ++n1;
+    }
+  }
+  if (n1 > SF_MAX_FUSED) return ctx->fail(SF_E_UNSUPPORTED, "a fused call takes at most %d 1D outputs", SF_MAX_FUSED);
+  // each output is validated as its own call would be
+  std::vector<std::vector<sf_problem>> per((size_t)n_out);
+  for (int k = 0; k < n_out; ++k) {
+    per[k].assign(sl, sl + std::max(n, 0));
+    for (sf_problem &q : per[k]) {
+      q.op = outs[k].op;
+      q.nl = outs[k].nl;
+      q.nt = outs[k].nt;
+      q.n_val = outs[k].n_val;
+      q.val_edges = outs[k].val_edges;
+      q.val_rule = outs[k].val_rule;
+    }
+    const int rc = validate_batch(ctx, n, per[k].data(), false);
+    if (rc) return rc;
+  }
+  if (n == 0) return SF_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  // binary64: one output after the other (the fused kernel computes in binary32)
+  if (sl[0].precision == SF_PREC_FP64) {
+    for (int k = 0; k < n_out; ++k) {
+      const int rc = sf_hist_batch_device(ctx, n, per[k].data(), d_sums[k], d_counts[k], st);
+      if (rc) return rc;
+    }
+    return SF_OK;
+  }
+  MultiSpec ms;
+  Plan &pl = ctx->fused;
+  for (int k = 0; k < n_out; ++k) {
+    if (k == joint) continue;
+    ms.op[ms.n1d] = outs[k].op;
+    ms.nl[ms.n1d] = outs[k].nl;
+    ms.nt[ms.n1d] = outs[k].nt;
+    pl.mdst_s[ms.n1d] = d_sums[k];
+    pl.mdst_c[ms.n1d] = (unsigned long long *)d_counts[k];
+    ++ms.n1d;
+  }
+  // the plan's main problem: the joint output's (its value bins), or the first 1D output's,
+  // whose own histogram then stays unused
+  const std::vector<sf_problem> &main = per[joint >= 0 ? joint : first1];
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax(n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+    if (sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
+      std::vector<double> hx((size_t)sl[s].W * sl[s].N);
+      HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double m = 0;
+      for (double v : hx) m = std::max(m, fabs(v));
+      xmax[s] = m;
+    }
+  }
+  int rc = plan_build(ctx, pl, n, main.data(), px.data(), pu.data(), xmax.data(), st, true, &ms);
+  if (rc) return rc;
+  for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
+  ctx->stats[0] = pl.total_pairs;
+  ctx->stats[2] = pl.n_items;
+  ctx->stats[3] = pl.grid;
+  ctx->stats[7] = pl.block;
+  ctx->ev_pending = false;
+  if (pl.total_pairs == 0) return SF_OK;
+  for (int k = 0; k < 4; ++k)
+    if (!ctx->ev[k]) HIPCHK(hipEventCreate(&ctx->ev[k]));
+  double *js = joint >= 0 ? d_sums[joint] : nullptr;
+  unsigned long long *jc = joint >= 0 ? (unsigned long long *)d_counts[joint] : nullptr;
+  unsigned long long pc = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    rc = plan_launch(pl, true, js, jc, st, ctx->ev);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(&pc, pl.pcount.p, sizeof(pc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats[1] = (int64_t)pc;
+    if ((int64_t)pc <= pl.cap || (int64_t)pc > kQueueMax) break;
+    // the merges were skipped: grow the re-bin queue to the size reached and run again
+    ctx->queue_hint = std::max<int64_t>(ctx->queue_hint, std::min<int64_t>((int64_t)pc + pc / 4, kQueueMax));
+    pl.cap = ctx->queue_hint;
+    HIPCHK(pl.pending.ensure((size_t)pl.cap));
+    pl.P.pending = pl.pending.p;
+    pl.P.pending_cap = pl.cap;
+  }
+  ctx->ev_pending = true;
+  if ((int64_t)pc > pl.cap) {
+    // still too many pairs to re-bin: every output with the binary64 kernel, one after the other
+    for (int k = 0; k < n_out; ++k) {
+      std::vector<sf_problem> q = per[k];
+      for (sf_problem &e : q) e.precision = SF_PREC_FP64;
+      rc = sf_hist_batch_device(ctx, n, q.data(), d_sums[k], d_counts[k], st);
+      if (rc) return rc;
+    }
+  }
+  return SF_OK;
+}
+
+// Host-pointer form: x, u and the outputs are host memory (staged like sf_hist_batch).
+extern "C" int sf_hist_multi(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out, const sf_output *outs,
+                             double *const *sums, uint64_t *const *counts) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  if (n < 0 || (n > 0 && !sl)) return ctx->fail(SF_E_ARG, "bad slices");
+  if (n_out < 1 || n_out > SF_MAX_FUSED + 1 || !outs || !sums || !counts)
+    return ctx->fail(SF_E_ARG, "a fused call needs 1 to %d outputs", SF_MAX_FUSED + 1);
+  {
+    // the host arrays are checked here; each output's operator and bins by sf_hist_multi_device
+    std::vector<sf_problem> chk(sl, sl + n);
+    for (sf_problem &q : chk) {
+      q.op = outs[0].op;
+      q.nl = outs[0].nl;
+      q.nt = outs[0].nt;
+      q.n_val = outs[0].n_val;
+      q.val_edges = outs[0].val_edges;
+      q.val_rule = outs[0].val_rule;
+    }
+    const int rc = validate_batch(ctx, n, chk.data(), true);
+    if (rc) return rc;
+  }
+  if (n == 0) return SF_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  int64_t tx = 0, tu = 0;
+  for (int s = 0; s < n; ++s) {
+    tx += (int64_t)sl[s].W * sl[s].N;
+    tu += (int64_t)sl[s].D * sl[s].N;
+  }
+  HIPCHK(ctx->d_x.ensure((size_t)tx + 2));
+  HIPCHK(ctx->d_u.ensure((size_t)tu + 2));
+  std::vector<sf_problem> dev(sl, sl + n);
+  int64_t ox = 0, ou = 0;
+  for (int s = 0; s < n; ++s) {
+    const int64_t nx = (int64_t)sl[s].W * sl[s].N, nu = (int64_t)sl[s].D * sl[s].N;
+    if (nx) HIPCHK(hipMemcpyAsync(ctx->d_x.p + ox, sl[s].x, sizeof(double) * nx, hipMemcpyHostToDevice, st));
+    if (nu) HIPCHK(hipMemcpyAsync(ctx->d_u.p + ou, sl[s].u, sizeof(double) * nu, hipMemcpyHostToDevice, st));
+    dev[s].x = ctx->d_x.p + ox;
+    dev[s].u = ctx->d_u.p + ou;
+    ox += nx;
+    ou += nu;
+  }
+  std::vector<DevBuf<double>> os((size_t)n_out);
+  std::vector<DevBuf<unsigned long long>> oc((size_t)n_out);
+  std::vector<double *> ds((size_t)n_out);
+  std::vector<uint64_t *> dc((size_t)n_out);
+  std::vector<int64_t> cells((size_t)n_out);
+  int rc = SF_OK;
+  for (int k = 0; k < n_out && !rc; ++k) {
+    cells[k] = (int64_t)n * sl[0].n_dist * (outs[k].n_val > 0 ? outs[k].n_val : 1);
+    if (os[k].ensure((size_t)cells[k]) != hipSuccess || oc[k].ensure((size_t)cells[k]) != hipSuccess)
+      rc = ctx->fail(SF_E_DEVICE, "output buffers");
+    else if (launch_zero(os[k].p, sizeof(double) * cells[k], st) != hipSuccess ||
+             launch_zero(oc[k].p, sizeof(unsigned long long) * cells[k], st) != hipSuccess)
+      rc = ctx->fail(SF_E_DEVICE, "output zeroing");
+    ds[k] = os[k].p;
+    dc[k] = (uint64_t *)oc[k].p;
+  }
+  if (!rc) rc = sf_hist_multi_device(ctx, n, dev.data(), n_out, outs, ds.data(), dc.data(), st);
+  for (int k = 0; k < n_out && !rc; ++k) {
+    std::vector<double> hs((size_t)cells[k]);
+    std::vector<unsigned long long> hc((size_t)cells[k]);
+    hipError_t e1 = hipMemcpyAsync(hs.data(), os[k].p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st);
+    hipError_t e2 = hipMemcpyAsync(hc.data(), oc[k].p, sizeof(unsigned long long) * hc.size(), hipMemcpyDeviceToHost, st);
+    hipError_t e3 = hipStreamSynchronize(st);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      rc = ctx->fail(SF_E_DEVICE, "device to host copy failed");
+      break;
+    }
+    for (int64_t c = 0; c < cells[k]; ++c) {
+      sums[k][c] += hs[c];
+      counts[k][c] += hc[c];
+    }
+  }
+  for (int k = 0; k < n_out; ++k) {
+    os[k].release();
+    oc[k].release();
+  }
+  return rc;
 }
 
 // ----------------------------------------------------------------------------------------------

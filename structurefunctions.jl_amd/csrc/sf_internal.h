@@ -76,6 +76,10 @@ struct Slice {
   // fixed-point joint accumulation: a value v of value bin g (0-based) is stored as the integer
   // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
   float vwf, vsf;
+  // ... except values with |v| < vsmall (a quarter bin width), whose fixed-point error would not be
+  // small relative to the value: they go to binary64 cells of the value bins gz .. gz + 2
+  float vsmall;
+  int32_t gz;
   int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
   float xerr;         // flat: 2^-28 max|x| (binary32 value bound: reference's direction error)
   int32_t strip_n;    // B-strip launch: fields of the strip this slice leads (0: not a leader)
@@ -131,7 +135,18 @@ struct Params {
   int64_t flush_limit;     // pair updates a workgroup may add to its LDS histogram between flushes
   int32_t bl;              // flat pairs follow the batch-leading kernels (SF_SEM_BATCH)
   int32_t strip;           // B-strip launch: LDS histograms per workgroup (fields per strip), 0: off
+  // fused outputs (one pair evaluation, several operators): n1d 1D histograms besides the joint
+  // one (if any); 1D output k of slice s accumulates at msums / mcounts + (k n_slices + s) n_dist
+  int32_t n1d;
+  int32_t mcopies;         // lane-group copies of each fused 1D histogram (power of two)
+  int32_t n_slices;
+  int32_t mop[8], mnl[8], mnt[8];
+  double *msums;
+  unsigned long long *mcounts;
 };
+
+// fused outputs: at most this many 1D operators per pair evaluation
+inline constexpr int kMaxFused = 8;
 
 // Fixed-point joint cell: one 64-bit word, the sum over its updates of kFxUnit + q, where
 // kFxUnit = 2^44 + kFxBase and q < 2^23 (value within [-1/64, 1 + 1/64] bin widths of a_g, scaled
@@ -154,8 +169,12 @@ inline constexpr __host__ __device__ int sp_row(int k) { return k < 2 ? k : k + 
 
 // LDS layout of the histogram kernels, shared by the host (sizing) and the device (carving).
 struct SmemLayout {
-  int32_t hs, hc, tab, dE, vE, vE32, stage, q, total;  // byte offsets
+  int32_t hs, hc, zs, zc, ms, mc, tab, dE, vE, vE32, stage, q, total;  // byte offsets
 };
+
+// fixed-point joint histograms: binary64 cells (sum, count) for values smaller than a quarter
+// bin width, kZeroBins value bins around zero per distance bin
+inline constexpr int kZeroBins = 3;
 
 // LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each).  1D
 // histograms keep one copy per lane group.  A joint histogram's slot of (distance bin d, value bin
@@ -201,7 +220,8 @@ inline constexpr int kStripMax = 4;
 // copies: 1D lane-group copies, or the joint histogram's copies (1 or 2); jlayout: joint layout
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
                                                   int stage_bytes, int copies, int nmom, int fx,
-                                                  int qcap, int strip = 0, int jlayout = 1) {
+                                                  int qcap, int strip = 0, int jlayout = 1, int n1d = 0,
+                                                  int mcopies = 1) {
   SmemLayout L;
   // + kDummySlots: one scratch slot per lane, target of unconditional atomics of lanes that have
   // nothing to add (never flushed); a B-strip launch keeps one histogram per field
@@ -210,6 +230,13 @@ inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int nce
   int o = 0;
   L.hs = o; o += 8 * nh * nmom;
   L.hc = o; o += fx ? 0 : 4 * ((nh + 3) & ~3);  // fixed-point cells carry their count
+  const int nz = fx ? kZeroBins * n_dist + kDummySlots : 0;
+  L.zs = o; o += 8 * nz;
+  L.zc = o; o += 4 * ((nz + 3) & ~3);
+  // fused 1D outputs: n1d histograms of n_dist x mcopies slots sharing one count per slot
+  const int nm = n1d > 0 ? n_dist * mcopies + kDummySlots : 0;
+  L.ms = o; o += 8 * nm * n1d;
+  L.mc = o; o += 4 * ((nm + 3) & ~3);
   L.tab = o; o += 16 * ncell_max;
   L.dE = o; o += 8 * ((n_dist + 2) & ~1);
   L.vE = o; o += 8 * ((n_val + 2) & ~1);

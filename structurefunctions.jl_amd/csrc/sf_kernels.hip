@@ -527,6 +527,13 @@ struct Smem {
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
   bool fx;                  // fixed-point joint cells (hq aliases hs; no hc)
   unsigned long long *hq;
+  double *zs;               // fixed-point joint: binary64 cells of small values (kZeroBins per
+  unsigned *zc;             // distance bin) and their counts
+  // fused 1D outputs: output k's sums at ms[k mstr + slot], shared counts mc[slot]; slot =
+  // distance bin * mcop + lane group; mscr is the lane's scratch slot
+  double *ms;
+  unsigned *mc;
+  int n1d, mcop, mstr, mscr;
   int qcap;                 // re-bin queue staged in LDS: entries in all
   int qw;                   // entries per wave: wave w owns q[w qw, (w + 1) qw) and fill qn[w]
   unsigned *qn;
@@ -535,7 +542,7 @@ struct Smem {
 
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx,
-                                   P.qcap, P.strip, P.jlayout);
+                                   P.qcap, P.strip, P.jlayout, P.n1d, P.mcopies);
   Smem m;
   m.nmom = P.nmom;
   m.fx = P.fx != 0;
@@ -556,6 +563,14 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.hs = (double *)(base + L.hs);
   m.hq = (unsigned long long *)(base + L.hs);
   m.hc = (unsigned *)(base + L.hc);
+  m.zs = (double *)(base + L.zs);
+  m.zc = (unsigned *)(base + L.zc);
+  m.ms = (double *)(base + L.ms);
+  m.mc = (unsigned *)(base + L.mc);
+  m.n1d = P.n1d;
+  m.mcop = P.mcopies > 0 ? P.mcopies : 1;
+  m.mstr = P.n_dist * m.mcop + kDummySlots;
+  m.mscr = P.n_dist * m.mcop + (int)__lane_id();
   m.tab = (Cell *)(base + L.tab);
   m.lohi = (float2 *)(base + L.tab);
   m.blo = (int *)(base + L.tab + 8 * P.ncell_max);
@@ -574,6 +589,38 @@ DEV void zero_hist(const Smem &m) {
   if (!m.fx)
     for (int c = threadIdx.x; c < m.mstride; c += blockDim.x) m.hc[c] = 0u;
   for (int c = threadIdx.x; c < m.mstride * m.nmom; c += blockDim.x) m.hs[c] = 0.0;
+  if (m.fx)
+    for (int c = threadIdx.x; c < kZeroBins * m.nd + kDummySlots; c += blockDim.x) {
+      m.zs[c] = 0.0;
+      m.zc[c] = 0u;
+    }
+  for (int c = threadIdx.x; c < m.mstr * m.n1d; c += blockDim.x) m.ms[c] = 0.0;
+  if (m.n1d > 0)
+    for (int c = threadIdx.x; c < m.mstr; c += blockDim.x) m.mc[c] = 0u;
+}
+
+// fused 1D outputs of slice slice_id: every output gets the shared counts and its own sums
+DEV void flush_multi(const Smem &m, const Params &P, int slice_id) {
+  for (int c = threadIdx.x; c < m.nd; c += blockDim.x) {
+    unsigned long long n = 0;
+    for (int k = 0; k < m.mcop; ++k) {
+      n += m.mc[c * m.mcop + k];
+      m.mc[c * m.mcop + k] = 0u;
+    }
+    for (int o = 0; o < m.n1d; ++o) {
+      double v = 0.0;
+      for (int k = 0; k < m.mcop; ++k) {
+        const int h = o * m.mstr + c * m.mcop + k;
+        v += m.ms[h];
+        m.ms[h] = 0.0;
+      }
+      if (n) {
+        const int64_t off = ((int64_t)o * P.n_slices + slice_id) * m.nd + c;
+        atomicAdd(&P.mcounts[off], n);
+        atomicAdd(&P.msums[off], v);
+      }
+    }
+  }
 }
 
 // LDS slot of output cell c = db0 + n_dist * vb0 (0-based bins) for this lane
@@ -623,6 +670,18 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
         const double v = (double)n * a + ((double)sq - (double)n * (double)kFxBias) / (double)s.vsf;
         const int64_t o = s.out_off + ostride * c;
         atomicAdd(&P.counts[o], n);
+        atomicAdd(&P.sums[o], v);
+      }
+    }
+    for (int c = threadIdx.x; c < kZeroBins * m.nd; c += blockDim.x) {
+      const unsigned n = m.zc[c];
+      const double v = m.zs[c];
+      m.zc[c] = 0u;
+      m.zs[c] = 0.0;
+      const int db0 = c / kZeroBins, vb0 = s.gz + c % kZeroBins;
+      if (n && vb0 >= 0 && vb0 < P.n_val) {
+        const int64_t o = s.out_off + ostride * (db0 + m.nd * vb0);
+        atomicAdd(&P.counts[o], (unsigned long long)n);
         atomicAdd(&P.sums[o], v);
       }
     }
@@ -943,7 +1002,9 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   constexpr bool SP = (OPK == 3);
   constexpr bool SP2 = (OPK == 4);
   constexpr bool needT = (OPK == 0) || (OPK >= 6) || ((SP || SP2) && G == kSphere);
-  constexpr bool needN2 = (OPK == 0) || SP || SP2;
+  constexpr bool needN2 = (OPK == 0) || (OPK == 9) || SP || SP2;
+  // OPK 9: fused outputs, the joint operator (P.op, if J) and P.n1d 1D operators per pair
+  constexpr bool MULTI = (OPK == 9);
   const int nb = P.n_dist;
   const bool upper = !MASK || j > i;
   bool amb, amb_d;
@@ -972,9 +1033,13 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // slot in pa.h) so that the wait below is a partial one on every path
     __builtin_amdgcn_sched_barrier(0);
     const int hh = pa.h;
-    if constexpr (FX) atomicAdd(&m.hq[hh], (1ull << kFxShift) | (unsigned long long)pa.q);
-    else if constexpr (SP) hist_add_sp(m, hh, pa.v, pa.w);
-    else hist_add(m, hh, pa.v);
+    if constexpr (FX) {
+      atomicAdd(&m.hq[hh], (1ull << kFxShift) | (unsigned long long)pa.q);
+    } else if constexpr (SP) {
+      hist_add_sp(m, hh, pa.v, pa.w);
+    } else {
+      hist_add(m, hh, pa.v);
+    }
     // consume the table entry here (an empty asm reading it, ordered before later memory
     // operations): the next column's record is requested only after this point, because a
     // scalar load still in flight would make the wait for the table read wait for it as well
@@ -1127,15 +1192,51 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   } else if constexpr (!SP) {
     amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   }
-  // binned here: certain and in range; queued: undecidable and not certainly out of range
-  const bool add = upper & !amb & db_ok & vb_ok;
+  // fused 1D outputs: every value must be finite in binary32 (else the pair is re-binned)
+  float mv[kMaxFused];
+  if constexpr (MULTI) {
+#pragma unroll
+    for (int k = 0; k < kMaxFused; ++k) {
+      mv[k] = 0.f;
+      if (k < P.n1d) {
+        mv[k] = op_value32(P.mop[k], P.mnl[k], P.mnt[k], D, L, T, tn2, n2);
+        amb = amb || !(__builtin_fabsf(mv[k]) < __builtin_inff());
+      }
+    }
+  }
+  // binned here: certain and in range; queued: undecidable and not certainly out of range.  A
+  // fused call without a joint output bins through its 1D histograms only.
+  const bool add = upper & !amb & db_ok & vb_ok & !(MULTI && !J);
   const bool defer = upper & amb & (amb_d | db_ok);
+  if constexpr (MULTI) {
+    // the fused 1D outputs of a certain, in-range distance bin (whatever the joint value bin)
+    const bool add1 = upper & !amb & db_ok;
+    const int h1 = add1 ? (db - 1) * m.mcop + (int)(__lane_id() & (m.mcop - 1)) : m.mscr;
+    atomicAdd(&m.mc[h1], 1u);
+#pragma unroll
+    for (int k = 0; k < kMaxFused; ++k)
+      if (k < P.n1d) atomicAdd(&m.ms[k * m.mstr + h1], (double)mv[k]);
+  }
   int h;
   // 24-bit multiplies (full rate): every operand of a slot that is used is a small bin index;
   // values of a pair that is not added may be garbage and are selected away below
   if constexpr (J) h = __mul24(vb0, m.jv) + __mul24(db, m.jd) + m.hoff;
   else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
-  pa.h = add ? h : m.scr;
+  if constexpr (FX) {
+    // a value below a quarter bin width goes to the binary64 cell of its (distance, value) bin
+    // at once (few lanes, so a branch): the fixed-point error (up to 2^-22 bin widths) would not
+    // be small relative to it.  The other pairs take the deferred fixed-point update.
+    const int k = vb0 - s.gz;
+    const bool small = add & (__builtin_fabsf(v) < s.vsmall) & ((unsigned)k < (unsigned)kZeroBins);
+    if (small) {
+      const int z = (db - 1) * kZeroBins + k;
+      atomicAdd(&m.zs[z], (double)v);
+      atomicAdd(&m.zc[z], 1u);
+    }
+    pa.h = (add & !small) ? h : m.scr;
+  } else {
+    pa.h = add ? h : m.scr;
+  }
   pa.v = SP ? L : v;
   pa.w = n2;
   // scalar-load variant: an empty asm that reads the last three words of the next record makes
@@ -1286,9 +1387,13 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
     }
     queue_deferred(P, m, dm, i, jg, slice_id);
   }
-  if constexpr (FX) atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
-  else if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
-  else hist_add(m, pa.h, pa.v);
+  if constexpr (FX) {
+    atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
+  } else if constexpr (OPK == 3) {
+    hist_add_sp(m, pa.h, pa.v, pa.w);
+  } else {
+    hist_add(m, pa.h, pa.v);
+  }
 }
 
 // the value-edge lookup (uniform or searched) is chosen once per column range, not per pair
@@ -1304,8 +1409,10 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
 // SLOAD: column records are read straight from global memory with wave-uniform addresses, which
 // the compiler turns into scalar loads (SGPR operands, scalar cache) -- no LDS staging, no
 // barriers inside an item.  Otherwise columns are staged through LDS in chunks of P.tj.
+// Fixed-point SLOAD kernels are held to 6 waves per SIMD (80 VGPRs): two 768-thread workgroups
+// per CU, which their small LDS footprint allows; the few values this spills are per-item ones.
 template <int G, int D, bool J, int OPK, bool SLOAD, bool FX>
-__global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict__ rec) {
+__global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
@@ -1327,6 +1434,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
     if (w.slice != cur || since_flush + work > (unsigned long long)P.flush_limit) {
       if (cur >= 0) {
         flush_hist(m, P, s);
+        if constexpr (OPK == 9) flush_multi(m, P, cur);
         __syncthreads();
       }
       if (w.slice != cur) {
@@ -1376,6 +1484,7 @@ __global__ void __launch_bounds__(1024) hist32(Params P, const float *__restrict
   if (cur >= 0) {
     queue_wave_flush(P, m, cur);
     flush_hist(m, P, s);
+    if constexpr (OPK == 9) flush_multi(m, P, cur);
   }
 }
 
@@ -1724,6 +1833,54 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
   }
 }
 
+// Fused outputs: binary64 re-binning of the queued pairs, every output evaluated exactly (the
+// joint operator into the joint accumulators, each 1D operator into its own) with global atomics
+// (the queue holds a small fraction of the pairs).
+template <int G, int D, bool J>
+__global__ void __launch_bounds__(256) recheck_multi(Params P) {
+  constexpr int S = Rec<G, D>::d64;
+  constexpr int F = Rec<G, D>::f32;
+  const unsigned long long n0 = *P.pending_count;
+  const unsigned long long n = n0 < (unsigned long long)P.pending_cap ? n0 : (unsigned long long)P.pending_cap;
+  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (unsigned long long)gridDim.x * blockDim.x) {
+    const Pending q = P.pending[k];
+    const Slice s = P.slices[q.slice];
+    const int ci = __float_as_int(P.rec32[(s.pt_off + q.i) * F + Rec<G, D>::id]);
+    const int cj = __float_as_int(P.rec32[(s.pt_off + q.j) * F + Rec<G, D>::id]);
+    const int64_t qi = ci < cj ? ci : cj, qj = ci < cj ? cj : ci;
+    double A[S], Bv[S];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      A[t] = P.rec64[(s.pt_off + qi) * S + t];
+      Bv[t] = P.rec64[(s.pt_off + qj) * S + t];
+    }
+    const Cell *tab = P.cells + s.tab_off;
+    const double *dE = P.dedges + s.dedge_off, *vE = P.vedges + s.vedge_off;
+    double v[2];
+    if constexpr (J) {
+      const int c = exact_cell<G, D, true, 0>(P, s, A, Bv, tab, dE, vE, v);
+      if (c >= 0) {
+        atomicAdd(&P.counts[s.out_off + c], 1ull);
+        atomicAdd(&P.sums[s.out_off + c], v[0]);
+      }
+    }
+    for (int o = 0; o < P.n1d; ++o) {
+      Params Q = P;
+      Q.op = P.mop[o];
+      Q.nl = P.mnl[o];
+      Q.nt = P.mnt[o];
+      Q.n_val = 0;
+      const int c = exact_cell<G, D, false, 0>(Q, s, A, Bv, tab, dE, vE, v);
+      if (c >= 0) {
+        const int64_t off = ((int64_t)o * P.n_slices + q.slice) * P.n_dist + c;
+        atomicAdd(&P.mcounts[off], 1ull);
+        atomicAdd(&P.msums[off], v[0]);
+      }
+    }
+  }
+}
+
 __global__ void merge_out(const double *src_s, const unsigned long long *src_c, double *dst_s,
                           unsigned long long *dst_c, int64_t n) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1823,6 +1980,8 @@ hipError_t launch_prep(int G, int D, bool mixed, int geom_code, const void *ps, 
 
 template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
+  // fused outputs: scalar-loaded columns only
+  if (opk == 9) return (const void *)&hist32<G, D, J, 9, true, FX>;
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
     if constexpr (SL) {  // 1D, scalar-loaded columns (the default): L^3, T, T^2, T^3
@@ -1871,6 +2030,7 @@ static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl,
 }
 
 static int opk_of(const Params &P) {
+  if (P.n1d > 0) return 9;
   if (sp2d(P.op, P.n_val)) return 4;
   if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
@@ -1930,7 +2090,19 @@ static const void *recheck_fn(bool J, bool sp) {
   return sp ? (const void *)&recheck<G, D, false, true> : (const void *)&recheck<G, D, false, false>;
 }
 
+template <int G, int D>
+static const void *recheck_multi_fn(bool J) {
+  return J ? (const void *)&recheck_multi<G, D, true> : (const void *)&recheck_multi<G, D, false>;
+}
+
 hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipStream_t st) {
+  if (P.n1d > 0) {
+    const void *f = G == kFlat ? (D == 2 ? recheck_multi_fn<kFlat, 2>(J) : recheck_multi_fn<kFlat, 3>(J))
+                               : (D == 2 ? recheck_multi_fn<kSphere, 2>(J) : recheck_multi_fn<kSphere, 3>(J));
+    Params Pc = P;
+    void *args[] = {&Pc};
+    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st);
+  }
   const bool sp = P.nmom > 1 || sp2d(P.op, P.n_val);
   const void *f;
   if (G == kFlat) f = D == 2 ? recheck_fn<kFlat, 2>(J, sp) : recheck_fn<kFlat, 3>(J, sp);

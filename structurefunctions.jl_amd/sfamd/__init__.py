@@ -7,7 +7,7 @@ libsfmi355x.so (HIP, gfx950) through its C ABI.
 from .bins import (AbstractBinEdges, BinEdges, InfPaddedBinEdges, LinearBinEdges, LogBinEdges,
                    LogBinEdges_from_log_edges, as_edges, midpoints, n_histogram_bins)
 from .dispatch import (MI355XBackend, calculate_structure_function, calculate_structure_function_batch,
-                       calculate_structure_function_inplace)
+                       calculate_structure_function_inplace, calculate_structure_functions_fused)
 from .errors import ArgumentError, BackendError, DimensionMismatch
 from .metrics import Euclidean, Haversine, SphericalAngle
 from .objects import (StructureFunction, StructureFunction2DSumsAndCounts, StructureFunctionSumsAndCounts,

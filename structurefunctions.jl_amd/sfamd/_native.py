@@ -25,7 +25,8 @@ EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batc
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_device_info", "sf_check",
             "sf_eval_math",
             "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy",
-            "sf_autotune", "sf_hist_batch_device_noalloc")
+            "sf_autotune", "sf_hist_batch_device_noalloc", "sf_hist_multi_device", "sf_hist_multi")
+SF_MAX_FUSED = 8
 
 
 class SfBinRule(ctypes.Structure):
@@ -55,6 +56,12 @@ class SfProblem(ctypes.Structure):
         ("semantics", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dist_rule", ctypes.c_void_p), ("val_rule", ctypes.c_void_p),
     ]
+
+
+class SfOutput(ctypes.Structure):
+    """sf_output: one operator of a fused call (n_val > 0: the joint histogram's value bins)."""
+    _fields_ = [("op", ctypes.c_int32), ("nl", ctypes.c_int32), ("nt", ctypes.c_int32),
+                ("n_val", ctypes.c_int32), ("val_edges", ctypes.c_void_p), ("val_rule", ctypes.c_void_p)]
 
 
 _lib = None
@@ -128,6 +135,12 @@ def lib():
     L.sf_autotune.restype = ctypes.c_int
     L.sf_hist_batch_device_noalloc.argtypes = [vp, i32, ctypes.POINTER(SfProblem), dp, dp, vp]
     L.sf_hist_batch_device_noalloc.restype = ctypes.c_int
+    L.sf_hist_multi_device.argtypes = [vp, i32, ctypes.POINTER(SfProblem), i32, ctypes.POINTER(SfOutput),
+                                       ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
+    L.sf_hist_multi_device.restype = ctypes.c_int
+    L.sf_hist_multi.argtypes = [vp, i32, ctypes.POINTER(SfProblem), i32, ctypes.POINTER(SfOutput),
+                                ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    L.sf_hist_multi.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -216,6 +229,24 @@ class Context:
         arr = (SfProblem * len(problems))(*problems)
         rc = lib().sf_hist_batch(self._h, len(problems), arr, sums.ctypes.data, counts.ctypes.data)
         self.check(rc)
+
+    def hist_multi(self, problems, outputs, sums_list, counts_list):
+        """sf_hist_multi: host-pointer fused call; sums_list[k] / counts_list[k] are contiguous
+        float64 / uint64 arrays of n_slices * ncell_k, accumulated into."""
+        arr = (SfProblem * len(problems))(*problems)
+        outs = (SfOutput * len(outputs))(*outputs)
+        ps = (ctypes.c_void_p * len(outputs))(*[a.ctypes.data for a in sums_list])
+        pc = (ctypes.c_void_p * len(outputs))(*[a.ctypes.data for a in counts_list])
+        self.check(lib().sf_hist_multi(self._h, len(problems), arr, len(outputs), outs, ps, pc))
+
+    def hist_multi_device(self, problems, outputs, d_sums_ptrs, d_counts_ptrs, stream_ptr: int = 0):
+        """sf_hist_multi_device: fused call on device pointers (one sums / counts address per output)."""
+        arr = (SfProblem * len(problems))(*problems)
+        outs = (SfOutput * len(outputs))(*outputs)
+        ps = (ctypes.c_void_p * len(outputs))(*[int(a) for a in d_sums_ptrs])
+        pc = (ctypes.c_void_p * len(outputs))(*[int(a) for a in d_counts_ptrs])
+        self.check(lib().sf_hist_multi_device(self._h, len(problems), arr, len(outputs), outs, ps, pc,
+                                              ctypes.c_void_p(stream_ptr) if stream_ptr else None))
 
     def hist_device_noalloc(self, problems, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0):
         """sf_hist_batch_device_noalloc: as hist_device, with no device allocation."""

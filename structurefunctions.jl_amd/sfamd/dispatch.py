@@ -7,6 +7,8 @@
         calculate_structure_function!  (dispatch.jl:346-373), accumulates
     calculate_structure_function_batch(sf, xs, us, distance_bins[, value_bins]; ...)
         ragged batches of independent point fields in one persistent launch
+    calculate_structure_functions_fused(outputs, x, u, distance_bins; ...)
+        several operators (and at most one joint histogram) from one evaluation of every pair
 
 Arrays follow the reference's shapes: ``x`` is (W, N) and ``u`` is (D, N) (or with trailing
 auxiliary axes, shapes.jl:67-105).  The only backend is :class:`MI355XBackend`, which calls the
@@ -237,18 +239,92 @@ def calculate_structure_function(sf, x, u, distance_bins, value_bins=None, count
     slices, aux = _slices(x, u, kind)
     sums, counts, nd, nv = _run(sf, slices, [de] * len(slices), [ve] * len(slices), distance_metric, backend,
                                 _semantics(kind), [dr] * len(slices), [vr] * len(slices))
-    B = len(slices)
-    if ve is None:
+    raw = _raw_result(sf, sums, counts, nd, nv, len(slices), aux, distance_bins, value_bins, count_eltype)
+    return _finalize(raw, output_type)
+
+
+def _raw_result(sf, sums, counts, nd, nv, B, aux, distance_bins, value_bins, count_eltype):
+    """Slice-major flat sums/counts (B slices of nd or nd x nv cells) as the reference's raw
+    result with the auxiliary axes last."""
+    if value_bins is None:
         s = sums.reshape(B, nd).T.reshape((nd,) + aux, order="F")
         c = counts.reshape(B, nd).T.reshape((nd,) + aux, order="F").astype(count_eltype)
-        raw = StructureFunctionSumsAndCounts(sf, distance_bins, s, c)
-    else:
-        s = np.stack([sums[b * nd * nv:(b + 1) * nd * nv].reshape((nd, nv), order="F") for b in range(B)], axis=-1)
-        c = np.stack([counts[b * nd * nv:(b + 1) * nd * nv].reshape((nd, nv), order="F") for b in range(B)], axis=-1)
-        s = s.reshape((nd, nv) + aux, order="F")
-        c = c.reshape((nd, nv) + aux, order="F").astype(count_eltype)
-        raw = StructureFunction2DSumsAndCounts(sf, distance_bins, value_bins, s, c)
-    return _finalize(raw, output_type)
+        return StructureFunctionSumsAndCounts(sf, distance_bins, s, c)
+    s = np.stack([sums[b * nd * nv:(b + 1) * nd * nv].reshape((nd, nv), order="F") for b in range(B)], axis=-1)
+    c = np.stack([counts[b * nd * nv:(b + 1) * nd * nv].reshape((nd, nv), order="F") for b in range(B)], axis=-1)
+    s = s.reshape((nd, nv) + aux, order="F")
+    c = c.reshape((nd, nv) + aux, order="F").astype(count_eltype)
+    return StructureFunction2DSumsAndCounts(sf, distance_bins, value_bins, s, c)
+
+
+def calculate_structure_functions_fused(outputs, x, u, distance_bins, count_eltype=np.uint32, *,
+                                        backend: MI355XBackend | None = None, output_type=None,
+                                        distance_metric=Euclidean()):
+    """Several pair histograms of one field from a single evaluation of every pair.
+
+    ``outputs`` lists operators; an entry ``(operator, value_bins)`` asks for that operator's joint
+    distance x value histogram (at most one such entry).  The result is the list of what
+    ``calculate_structure_function(operator, x, u, distance_bins[, value_bins])`` returns for each
+    entry, in order: counts identical, sums to the backend precision's bound.  The reference
+    computes each operator with its own call (dispatch.jl:64-136); this form shares the pair
+    geometry, the longitudinal/transverse frame and the increments across the outputs
+    (sfmi355x.h, sf_hist_multi).  At most 8 one-dimensional outputs plus one joint output.
+    """
+    entries = list(outputs)
+    if not entries:
+        raise ArgumentError("outputs must name at least one operator")
+    ops, vbins = [], []
+    for e in entries:
+        sf, vb = (e if isinstance(e, tuple) else (e, None))
+        if isinstance(sf, AbstractDerivedStructureFunctionType):
+            _derived_error(sf)
+        if not isinstance(sf, AbstractPairwiseStructureFunctionType):
+            raise ArgumentError(f"{sf!r} is not a structure-function operator")
+        if sf.code[0] == _native.OP_SINGLE_PASS:
+            raise ArgumentError("the single pass is its own fused form; call "
+                                "calculate_structure_functions_single_pass instead")
+        ops.append(sf)
+        vbins.append(vb)
+    if sum(v is not None for v in vbins) > 1:
+        raise ArgumentError("a fused call takes at most one joint (operator, value_bins) output")
+    if sum(v is None for v in vbins) > _native.SF_MAX_FUSED:
+        raise ArgumentError(f"a fused call takes at most {_native.SF_MAX_FUSED} one-dimensional outputs")
+    backend = backend or MI355XBackend()
+    x = np.asarray(x, dtype=np.float64)
+    u = np.asarray(u, dtype=np.float64)
+    kind, D, W = _validate_array_shape(x, u, distance_metric)
+    _assert_counts_representable(count_eltype, x.shape[1])
+    de = as_edges(distance_bins)
+    dr = rule_of(distance_bins)
+    slices, aux = _slices(x, u, kind)
+    keep = []
+    problems = [_problem(ops[0], xs, us, de, None, distance_metric, backend.precision_code, keep,
+                         _semantics(kind), dr, None) for xs, us in slices]
+    nd = len(de) - 1
+    outs, sums, counts, nvs = [], [], [], []
+    for sf, vb in zip(ops, vbins):
+        o = _native.SfOutput()
+        o.op, o.nl, o.nt = sf.code
+        nv = 0
+        if vb is not None:
+            ve = as_edges(vb)
+            keep.append(ve)
+            nv = len(ve) - 1
+            o.n_val, o.val_edges = nv, ve.ctypes.data
+            r = _native.make_rule(rule_of(vb))
+            if r is not None:
+                keep.append(r)
+                o.val_rule = ctypes.addressof(r)
+        outs.append(o)
+        nvs.append(nv)
+        sums.append(np.zeros(len(slices) * nd * max(nv, 1), dtype=np.float64))
+        counts.append(np.zeros(len(slices) * nd * max(nv, 1), dtype=np.uint64))
+    backend.context().hist_multi(problems, outs, sums, counts)
+    res = []
+    for sf, vb, nv, s, c in zip(ops, vbins, nvs, sums, counts):
+        raw = _raw_result(sf, s, c, nd, nv, len(slices), aux, distance_bins, vb, count_eltype)
+        res.append(_finalize(raw, output_type if vb is None else None))
+    return res
 
 
 def calculate_structure_function_inplace(sums, counts, sf, x, u, distance_bins, value_bins=None, *,
