@@ -624,10 +624,12 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
           s.v_fx = 1;
           s.vwf = (float)w;
           s.vsf = kFxScale / s.vwf;
-          // values with |v| < w/4 lie in the value bins floor((-w/4 - v0)/w) and the next; one
-          // more bin below covers rounding
+          // a certified bin g of a value with |v| < w/4 is within half a bin of [-w/4, w/4] (the
+          // binary32 value's error is below half a bin, or the pair is re-binned), so g - gz is
+          // 1 .. 3 with gz = floor((-w/4 - v0)/w) - 2; the kZeroBins = 5 cells gz .. gz + 4 leave
+          // a bin of margin on either side
           s.vsmall = (float)(0.25L * w);
-          const long double gzl = floorl((-0.25L * w - (long double)v0) / w) - 1.0L;
+          const long double gzl = floorl((-0.25L * w - (long double)v0) / w) - 2.0L;
           s.gz = (int32_t)std::max(-1.0e9L, std::min(1.0e9L, gzl));
         }
       }
@@ -1090,11 +1092,13 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     copies = 2;  // joint histogram privatised by lane halves (binary32 kernel only)
   }
   const int jlayout = (J && tn.layout) ? tn.layout : 1;
-  // fused 1D outputs: lane-group copies of their histograms (fewer beside a joint histogram, so
-  // that the LDS still holds two workgroups per CU)
+  // fused 1D outputs: 16 lane-group copies of their histograms (lanes l, l + 16, l + 32, l + 48
+  // share one), fewer only if the LDS cannot hold them.  Their binary64 LDS atomics bound the
+  // fused kernel (one per output and pair): beside the sphere's 64 x 96 joint histogram, 4, 8, 16
+  // and 32 copies measured 199, 176, 169 and 169 ms for four 1D outputs at N = 262144
   int mcop = 0;
   if (pl.n1d) {
-    mcop = J ? 4 : 16;
+    mcop = 16;
     while (mcop > 1 && (size_t)smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0,
                                            strip, jlayout, pl.n1d, mcop).total > ctx->max_smem)
       mcop >>= 1;
@@ -1310,7 +1314,16 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     P.mop[k] = multi->op[k];
     P.mnl[k] = multi->nl[k];
     P.mnt[k] = multi->nt[k];
+    int a = 0, b = 0;
+    if (!fused_basis_index(multi->op[k], multi->nl[k], multi->nt[k], &a, &b))
+      return ctx->fail(SF_E_DEVICE, "internal: fused output %d is not a basis product", k);
+    P.ma[k] = a;
+    P.mb[k] = b;
   }
+  for (int k = pl.n1d; k < kMaxFused; ++k) P.ma[k] = P.mb[k] = 0;  // unused outputs: 1 * 1
+  P.ja = P.jb = 0;
+  if (pl.n1d && J && !fused_basis_index(p0.op, p0.nl, p0.nt, &P.ja, &P.jb))
+    return ctx->fail(SF_E_DEVICE, "internal: the fused joint operator is not a basis product");
   P.msums = pl.n1d ? pl.msums.p : nullptr;
   P.mcounts = pl.n1d ? pl.mcounts.p : nullptr;
   P.flush_limit = pl.fx ? kFxFlushLimit : 0x7fffffffLL;
@@ -1589,8 +1602,25 @@ extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
   }
   MultiSpec ms;
   Plan &pl = ctx->fused;
+  // 1D outputs whose binary32 value is not a product of two basis values (FullVector{1}, {3}:
+  // a square root per pair) run as their own calls after the fused launch
+  std::vector<int> separate;
+  first1 = -1;
+  {
+    int a = 0, b = 0;
+    if (joint >= 0 && !fused_basis_index(outs[joint].op, outs[joint].nl, outs[joint].nt, &a, &b)) {
+      separate.push_back(joint);
+      joint = -1;
+    }
+  }
   for (int k = 0; k < n_out; ++k) {
-    if (k == joint) continue;
+    if (k == joint || (outs[k].n_val > 0 && joint < 0)) continue;
+    int a = 0, b = 0;
+    if (!fused_basis_index(outs[k].op, outs[k].nl, outs[k].nt, &a, &b)) {
+      separate.push_back(k);
+      continue;
+    }
+    if (first1 < 0) first1 = k;
     ms.op[ms.n1d] = outs[k].op;
     ms.nl[ms.n1d] = outs[k].nl;
     ms.nt[ms.n1d] = outs[k].nt;
@@ -1598,6 +1628,11 @@ extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
     pl.mdst_c[ms.n1d] = (unsigned long long *)d_counts[k];
     ++ms.n1d;
   }
+  for (int k : separate) {
+    const int rc = sf_hist_batch_device(ctx, n, per[k].data(), d_sums[k], d_counts[k], st);
+    if (rc) return rc;
+  }
+  if (joint < 0 && first1 < 0) return SF_OK;
   // the plan's main problem: the joint output's (its value bins), or the first 1D output's,
   // whose own histogram then stays unused
   const std::vector<sf_problem> &main = per[joint >= 0 ? joint : first1];
@@ -1645,8 +1680,10 @@ extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl
   }
   ctx->ev_pending = true;
   if ((int64_t)pc > pl.cap) {
-    // still too many pairs to re-bin: every output with the binary64 kernel, one after the other
+    // still too many pairs to re-bin: every fused output with the binary64 kernel, one after
+    // the other (the separate outputs are done)
     for (int k = 0; k < n_out; ++k) {
+      if (std::find(separate.begin(), separate.end(), k) != separate.end()) continue;
       std::vector<sf_problem> q = per[k];
       for (sf_problem &e : q) e.precision = SF_PREC_FP64;
       rc = sf_hist_batch_device(ctx, n, q.data(), d_sums[k], d_counts[k], st);

@@ -77,7 +77,7 @@ struct Slice {
   // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
   float vwf, vsf;
   // ... except values with |v| < vsmall (a quarter bin width), whose fixed-point error would not be
-  // small relative to the value: they go to binary64 cells of the value bins gz .. gz + 2
+  // small relative to the value: they go to binary64 cells of the value bins gz .. gz + kZeroBins - 1
   float vsmall;
   int32_t gz;
   int32_t plane;      // single pass, 2D form: invariant (0..5) whose joint histogram this slice holds
@@ -141,12 +141,33 @@ struct Params {
   int32_t mcopies;         // lane-group copies of each fused 1D histogram (power of two)
   int32_t n_slices;
   int32_t mop[8], mnl[8], mnt[8];
+  // binary32 value of fused 1D output k: X[ma[k]] * X[mb[k]] over the per-pair basis
+  // X = {1, du_L, du_L^2, du_L^3, du_T, tn2, du_T^3, |du|^2, tn2 / (D - 1)} (fused_basis_index)
+  int32_t ma[8], mb[8];
+  int32_t ja, jb;          // the joint operator's basis indices (fused plan with a joint output)
   double *msums;
   unsigned long long *mcounts;
 };
 
 // fused outputs: at most this many 1D operators per pair evaluation
 inline constexpr int kMaxFused = 8;
+
+// Basis indices (a, b) of a fused 1D operator, whose binary32 value op_value32 forms as the
+// product X[a] * X[b] with the same operations (x * 1 and 1 * x are exact): du_L^nl * (1, du_T,
+// tn2, du_T^3 for nt = 0..3); |du|^2 (S2, FullVector{2}); du_L |du|^2 (S3); tn2 / (D - 1) and
+// du_L tn2 / (D - 1) (the component forms).  FullVector{1} and {3} take a square root: not
+// fusable (returns false).
+inline bool fused_basis_index(int op, int nl, int nt, int *a, int *b) {
+  switch (op) {
+    case 0: *a = nl; *b = nt == 0 ? 0 : 3 + nt; return nl >= 0 && nl <= 3 && nt >= 0 && nt <= 3;
+    case 1: *a = 0; *b = 7; return true;
+    case 2: *a = 1; *b = 7; return true;
+    case 3: *a = 0; *b = 8; return true;
+    case 4: *a = 1; *b = 8; return true;
+    case 5: *a = 0; *b = 7; return nl == 2;
+    default: return false;
+  }
+}
 
 // Fixed-point joint cell: one 64-bit word, the sum over its updates of kFxUnit + q, where
 // kFxUnit = 2^44 + kFxBase and q < 2^23 (value within [-1/64, 1 + 1/64] bin widths of a_g, scaled
@@ -174,7 +195,7 @@ struct SmemLayout {
 
 // fixed-point joint histograms: binary64 cells (sum, count) for values smaller than a quarter
 // bin width, kZeroBins value bins around zero per distance bin
-inline constexpr int kZeroBins = 3;
+inline constexpr int kZeroBins = 5;
 
 // LDS histogram slots (nmom binary64 sums, moment-major, and one 32-bit count each).  1D
 // histograms keep one copy per lane group.  A joint histogram's slot of (distance bin d, value bin

@@ -960,7 +960,12 @@ DEV float op32(const Params &P, int D, float L, float T, float tn2, float n2) {
 // (|u_i| + |u_j|)^order without a loop (order 0..3)
 template <int OPK>
 DEV float pow_order(const Params &P, float x) {
-  if constexpr (OPK == 1 || OPK == 6) return x;
+  if constexpr (OPK >= 9) {
+    // fused: the joint operator's order, selected without a branch
+    const float x2 = x * x, x3 = x2 * x;
+    const float t = P.order >= 2 ? x2 : (P.order == 1 ? x : 1.0f);
+    return P.order == 3 ? x3 : t;
+  } else if constexpr (OPK == 1 || OPK == 6) return x;
   else if constexpr (OPK == 2 || OPK == 7) return x * x;
   else if constexpr (OPK == 5 || OPK == 8) return x * x * x;
   else {
@@ -1002,9 +1007,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   constexpr bool SP = (OPK == 3);
   constexpr bool SP2 = (OPK == 4);
   constexpr bool needT = (OPK == 0) || (OPK >= 6) || ((SP || SP2) && G == kSphere);
-  constexpr bool needN2 = (OPK == 0) || (OPK == 9) || SP || SP2;
-  // OPK 9: fused outputs, the joint operator (P.op, if J) and P.n1d 1D operators per pair
-  constexpr bool MULTI = (OPK == 9);
+  // OPK 9, 10, 11: fused outputs -- the joint operator (if J) and P.n1d <= NF 1D operators per
+  // pair, every value a product of two basis values (fused_basis_index)
+  constexpr bool MULTI = (OPK >= 9);
+  constexpr int NF = OPK == 9 ? 2 : (OPK == 10 ? 4 : kMaxFused);
+  constexpr bool needN2 = (OPK == 0) || MULTI || SP || SP2;
   const int nb = P.n_dist;
   const bool upper = !MASK || j > i;
   bool amb, amb_d;
@@ -1139,6 +1146,15 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
       dfloor = umax * __builtin_fmaf(0x1p-26f, inv_s, 0x1p-20f);
     }
   }
+  // fused outputs: the basis X = {1, du_L, du_L^2, du_L^3, du_T, tn2, du_T^3, |du|^2,
+  // tn2 / (D - 1)}; each value is X[a] * X[b] with wave-uniform a, b (nine entries: the compiler
+  // reads them with register indexing, one move each, no compare chain)
+  float X[MULTI ? 9 : 1];
+  if constexpr (MULTI) {
+    const float L2 = L * L;
+    X[0] = 1.0f; X[1] = L; X[2] = L2; X[3] = L2 * L; X[4] = T; X[5] = tn2; X[6] = (T * T) * T;
+    X[7] = n2; X[8] = (D == 3) ? tn2 * 0.5f : tn2;
+  }
   if constexpr (SP) {
     // moments are formed at the accumulation; all four must be finite in binary32
     const float L2 = L * L;
@@ -1146,6 +1162,8 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
           !(__builtin_fabsf(L * L2) < __builtin_inff());
   } else if constexpr (SP2) {
     v = sp_invariant(s.plane, L, n2);
+  } else if constexpr (MULTI) {
+    v = X[P.ja] * X[P.jb];
   } else {
     v = op32<OPK>(P, D, L, T, tn2, n2);
   }
@@ -1193,16 +1211,18 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   }
   // fused 1D outputs: every value must be finite in binary32 (else the pair is re-binned)
-  float mv[kMaxFused];
+  // fused 1D outputs: NF values per pair, unconditionally (outputs past P.n1d have a = b = 0)
+  float mv[MULTI ? NF : 1];
   if constexpr (MULTI) {
+    // one finiteness test for all: a sum of magnitudes is finite iff every term is (a sum that
+    // overflows from finite terms only re-bins the pair in binary64)
+    float mag = 0.f;
 #pragma unroll
-    for (int k = 0; k < kMaxFused; ++k) {
-      mv[k] = 0.f;
-      if (k < P.n1d) {
-        mv[k] = op_value32(P.mop[k], P.mnl[k], P.mnt[k], D, L, T, tn2, n2);
-        amb = amb || !(__builtin_fabsf(mv[k]) < __builtin_inff());
-      }
+    for (int k = 0; k < NF; ++k) {
+      mv[k] = X[P.ma[k]] * X[P.mb[k]];
+      mag += __builtin_fabsf(mv[k]);
     }
+    amb = amb || !(mag < __builtin_inff());
   }
   // binned here: certain and in range; queued: undecidable and not certainly out of range.  A
   // fused call without a joint output bins through its 1D histograms only.
@@ -1214,7 +1234,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const int h1 = add1 ? (db - 1) * m.mcop + (int)(__lane_id() & (m.mcop - 1)) : m.mscr;
     atomicAdd(&m.mc[h1], 1u);
 #pragma unroll
-    for (int k = 0; k < kMaxFused; ++k)
+    for (int k = 0; k < NF; ++k)
       if (k < P.n1d) atomicAdd(&m.ms[k * m.mstr + h1], (double)mv[k]);
   }
   int h;
@@ -1225,9 +1245,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   if constexpr (FX) {
     // a value below a quarter bin width goes to the binary64 cell of its (distance, value) bin
     // at once (few lanes, so a branch): the fixed-point error (up to 2^-22 bin widths) would not
-    // be small relative to it.  The other pairs take the deferred fixed-point update.
-    const int k = vb0 - s.gz;
-    const bool small = add & (__builtin_fabsf(v) < s.vsmall) & ((unsigned)k < (unsigned)kZeroBins);
+    // be small relative to it.  The other pairs take the deferred fixed-point update.  The clamp
+    // keeps the cell in range; the host's choice of gz already puts every such bin inside it
+    // (a range test in its place measured 0.8 % slower)
+    const int k = min(max(vb0 - s.gz, 0), kZeroBins - 1);
+    const bool small = add & (__builtin_fabsf(v) < s.vsmall);
     if (small) {
       const int z = (db - 1) * kZeroBins + k;
       atomicAdd(&m.zs[z], (double)v);
@@ -1412,7 +1434,7 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
 // Fixed-point SLOAD kernels are held to 6 waves per SIMD (80 VGPRs): two 768-thread workgroups
 // per CU, which their small LDS footprint allows; the few values this spills are per-item ones.
 template <int G, int D, bool J, int OPK, bool SLOAD, bool FX>
-__global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
+__global__ void __launch_bounds__(1024, (FX && SLOAD && OPK < 9) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
@@ -1434,7 +1456,7 @@ __global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, 
     if (w.slice != cur || since_flush + work > (unsigned long long)P.flush_limit) {
       if (cur >= 0) {
         flush_hist(m, P, s);
-        if constexpr (OPK == 9) flush_multi(m, P, cur);
+        if constexpr (OPK >= 9) flush_multi(m, P, cur);
         __syncthreads();
       }
       if (w.slice != cur) {
@@ -1484,7 +1506,7 @@ __global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, 
   if (cur >= 0) {
     queue_wave_flush(P, m, cur);
     flush_hist(m, P, s);
-    if constexpr (OPK == 9) flush_multi(m, P, cur);
+    if constexpr (OPK >= 9) flush_multi(m, P, cur);
   }
 }
 
@@ -1834,16 +1856,32 @@ __global__ void __launch_bounds__(256) recheck(Params P) {
 }
 
 // Fused outputs: binary64 re-binning of the queued pairs, every output evaluated exactly (the
-// joint operator into the joint accumulators, each 1D operator into its own) with global atomics
-// (the queue holds a small fraction of the pairs).
+// joint operator into the joint accumulators, each 1D operator into its own).  Like recheck, each
+// workgroup takes a contiguous range of the queue and accumulates the entries of the range's
+// first slice in LDS histograms flushed once (the 1D outputs share one count per distance bin);
+// entries of other slices go straight to the global accumulators.
 template <int G, int D, bool J>
 __global__ void __launch_bounds__(256) recheck_multi(Params P) {
+  extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
   constexpr int F = Rec<G, D>::f32;
+  const int nh = J ? P.n_dist * P.n_val : 0;  // joint cells
+  const int nd = P.n_dist, n1 = P.n1d;
+  double *hs = (double *)smem_raw;            // joint sums [nh], then 1D sums [n1][nd]
+  double *ms = hs + nh;
+  unsigned *hc = (unsigned *)(ms + n1 * nd);  // joint counts [nh], then 1D counts [nd]
+  unsigned *mc = hc + nh;
+  for (int c = threadIdx.x; c < nh + nd; c += blockDim.x) hc[c] = 0u;
+  for (int c = threadIdx.x; c < nh + n1 * nd; c += blockDim.x) hs[c] = 0.0;
   const unsigned long long n0 = *P.pending_count;
   const unsigned long long n = n0 < (unsigned long long)P.pending_cap ? n0 : (unsigned long long)P.pending_cap;
-  for (unsigned long long k = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (unsigned long long)gridDim.x * blockDim.x) {
+  const unsigned long long per = (n + gridDim.x - 1) / gridDim.x;
+  const unsigned long long beg = (unsigned long long)blockIdx.x * per;
+  const unsigned long long end = beg + per < n ? beg + per : n;
+  __syncthreads();
+  if (beg >= end) return;
+  const int home = P.pending[beg].slice;
+  for (unsigned long long k = beg + threadIdx.x; k < end; k += blockDim.x) {
     const Pending q = P.pending[k];
     const Slice s = P.slices[q.slice];
     const int ci = __float_as_int(P.rec32[(s.pt_off + q.i) * F + Rec<G, D>::id]);
@@ -1857,15 +1895,21 @@ __global__ void __launch_bounds__(256) recheck_multi(Params P) {
     }
     const Cell *tab = P.cells + s.tab_off;
     const double *dE = P.dedges + s.dedge_off, *vE = P.vedges + s.vedge_off;
+    const bool local = (q.slice == home);
     double v[2];
     if constexpr (J) {
       const int c = exact_cell<G, D, true, 0>(P, s, A, Bv, tab, dE, vE, v);
       if (c >= 0) {
-        atomicAdd(&P.counts[s.out_off + c], 1ull);
-        atomicAdd(&P.sums[s.out_off + c], v[0]);
+        if (local) {
+          atomicAdd(&hc[c], 1u);
+          atomicAdd(&hs[c], v[0]);
+        } else {
+          atomicAdd(&P.counts[s.out_off + c], 1ull);
+          atomicAdd(&P.sums[s.out_off + c], v[0]);
+        }
       }
     }
-    for (int o = 0; o < P.n1d; ++o) {
+    for (int o = 0; o < n1; ++o) {
       Params Q = P;
       Q.op = P.mop[o];
       Q.nl = P.mnl[o];
@@ -1873,10 +1917,30 @@ __global__ void __launch_bounds__(256) recheck_multi(Params P) {
       Q.n_val = 0;
       const int c = exact_cell<G, D, false, 0>(Q, s, A, Bv, tab, dE, vE, v);
       if (c >= 0) {
-        const int64_t off = ((int64_t)o * P.n_slices + q.slice) * P.n_dist + c;
-        atomicAdd(&P.mcounts[off], 1ull);
-        atomicAdd(&P.msums[off], v[0]);
+        if (local) {
+          if (o == 0) atomicAdd(&mc[c], 1u);  // the distance bin is the same for every output
+          atomicAdd(&ms[o * nd + c], v[0]);
+        } else {
+          const int64_t off = ((int64_t)o * P.n_slices + q.slice) * nd + c;
+          atomicAdd(&P.mcounts[off], 1ull);
+          atomicAdd(&P.msums[off], v[0]);
+        }
       }
+    }
+  }
+  __syncthreads();
+  const int64_t joff = P.slices[home].out_off;
+  for (int c = threadIdx.x; c < nh; c += blockDim.x)
+    if (hc[c]) {
+      atomicAdd(&P.counts[joff + c], (unsigned long long)hc[c]);
+      atomicAdd(&P.sums[joff + c], hs[c]);
+    }
+  for (int c = threadIdx.x; c < n1 * nd; c += blockDim.x) {
+    const int o = c / nd, b = c - o * nd;
+    if (mc[b]) {
+      const int64_t off = ((int64_t)o * P.n_slices + home) * nd + b;
+      atomicAdd(&P.mcounts[off], (unsigned long long)mc[b]);
+      atomicAdd(&P.msums[off], ms[c]);
     }
   }
 }
@@ -1982,6 +2046,8 @@ template <int G, int D, bool J, bool SL, bool FX>
 static const void *hist32_fn(int opk) {
   // fused outputs: scalar-loaded columns only
   if (opk == 9) return (const void *)&hist32<G, D, J, 9, true, FX>;
+  if (opk == 10) return (const void *)&hist32<G, D, J, 10, true, FX>;
+  if (opk == 11) return (const void *)&hist32<G, D, J, 11, true, FX>;
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
     if constexpr (SL) {  // 1D, scalar-loaded columns (the default): L^3, T, T^2, T^3
@@ -2030,7 +2096,7 @@ static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl,
 }
 
 static int opk_of(const Params &P) {
-  if (P.n1d > 0) return 9;
+  if (P.n1d > 0) return P.n1d <= 2 ? 9 : (P.n1d <= 4 ? 10 : 11);
   if (sp2d(P.op, P.n_val)) return 4;
   if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;
@@ -2101,7 +2167,10 @@ hipError_t launch_recheck(int G, int D, bool J, const Params &P, int grid, hipSt
                                : (D == 2 ? recheck_multi_fn<kSphere, 2>(J) : recheck_multi_fn<kSphere, 3>(J));
     Params Pc = P;
     void *args[] = {&Pc};
-    return hipLaunchKernel(f, dim3(grid), dim3(256), args, 0, st);
+    const size_t nh = J ? (size_t)P.n_dist * P.n_val : 0;
+    const size_t smem = nh * 12 + (size_t)P.n1d * P.n_dist * 8 + (size_t)P.n_dist * 4 + 16;
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    return hipLaunchKernel(f, dim3(grid), dim3(256), args, smem, st);
   }
   const bool sp = P.nmom > 1 || sp2d(P.op, P.n_val);
   const void *f;
