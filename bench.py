@@ -44,7 +44,12 @@ TABLE_FLOPS = {("cfg1", "L2"): 28,
                ("cfg4", "joint L1"): 60,
                # the six single-pass invariants from one pair evaluation: flat 2D 40; sphere: the
                # common spherical part 43, du_L 8, du_T 8, |du|^2 and the four moments 10
-               ("cfg2", "single pass"): 40, ("cfg4", "single pass"): 69}
+               ("cfg2", "single pass"): 40, ("cfg4", "single pass"): 69,
+               # fused outputs (one pair evaluation, sf_hist_multi_device), SURVEY-style counts:
+               # flat 2D all six projected operators 40; sphere four 1D + joint du_L 76
+               ("cfg2", "fused L1 T1 L2 T2 L3 T3"): 40, ("cfg4", "fused L2 T2 L3 T3 + joint L1"): 76}
+FUSED = {"fused L1 T1 L2 T2 L3 T3": ["L1", "T1", "L2", "T2", "L3", "T3"],
+         "fused L2 T2 L3 T3 + joint L1": ["L2", "T2", "L3", "T3", "joint L1"]}
 OPS = {"L1": (0, 1, 0), "T1": (0, 0, 1), "L2": (0, 2, 0), "T2": (0, 0, 2), "L3": (0, 3, 0),
        "T3": (0, 0, 3), "joint L1": (0, 1, 0), "single pass": (6, 0, 0)}
 PEAK_FP32_TFLOPS = 157.3
@@ -511,12 +516,12 @@ def bench_table(args):
     cases.append(("cfg1", "flat 2D N=4096", x1, u1, e1, None, 0, ["L2"]))
     x2, u2, e2 = synthetic.config2()
     cases.append(("cfg2", "flat 2D N=65536", x2, u2, e2, None, 0,
-                  ["L1", "T1", "L2", "T2", "L3", "T3", "single pass"]))
+                  ["L1", "T1", "L2", "T2", "L3", "T3", "single pass", "fused L1 T1 L2 T2 L3 T3"]))
     x3, u3, e3, v3 = synthetic.config3()
     cases.append(("cfg3", "flat 3D N=131072, 48x128", x3, u3, e3, v3, 0, ["joint L1"]))
     x4, u4, e4, v4 = synthetic.config4()
     cases.append(("cfg4", "sphere N=262144 (degrees, Haversine(1)), 64 distance bins", x4, u4, e4, v4, 1,
-                  ["L2", "T2", "L3", "T3", "joint L1", "single pass"]))
+                  ["L2", "T2", "L3", "T3", "joint L1", "single pass", "fused L2 T2 L3 T3 + joint L1"]))
     rows = []
     for cfg, desc, x, u, de, ve, geom, ops in cases:
         N, D, W = x.shape[1], u.shape[0], x.shape[0]
@@ -524,20 +529,35 @@ def bench_table(args):
         dx = torch.from_numpy(np.ascontiguousarray(x.T)).to(dev)
         du = torch.from_numpy(np.ascontiguousarray(u.T)).to(dev)
         for name in ops:
-            joint = name.startswith("joint")
+            fused = FUSED.get(name)
+            joint = name.startswith("joint") or (fused is not None and "joint L1" in fused)
             vedges = ve if joint else None
             ncell = (len(de) - 1) * ((len(ve) - 1) if joint else (6 if name == "single pass" else 1))
             s = torch.zeros(ncell, dtype=torch.float64, device=dev)
             c = torch.zeros(ncell, dtype=torch.int64, device=dev)
+            outs, bufs = [], []
+            for o in fused or []:
+                so = _native.SfOutput()
+                so.op, so.nl, so.nt = OPS[o]
+                if o == "joint L1":
+                    so.n_val, so.val_edges = len(ve) - 1, ve.ctypes.data
+                outs.append(so)
+                n_o = (len(de) - 1) * ((len(ve) - 1) if o == "joint L1" else 1)
+                bufs.append((torch.zeros(n_o, dtype=torch.float64, device=dev),
+                             torch.zeros(n_o, dtype=torch.int64, device=dev)))
             precs = ["mixed_fixed", "mixed"] if joint else ["mixed"]
             for prec in precs:
-                p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, D, W, geom, OPS[name], de,
-                                    vedges, prec)
+                p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, D, W, geom,
+                                    OPS[fused[0]] if fused else OPS[name], de, None if fused else vedges, prec)
                 best_k = best_w = None
                 for r in range(max(1, args.steps) + 1):
                     torch.cuda.synchronize()
                     t0 = time.perf_counter()
-                    ctx.hist_device([p], s.data_ptr(), c.data_ptr(), stream)
+                    if fused:
+                        ctx.hist_multi_device([p], outs, [b[0].data_ptr() for b in bufs],
+                                              [b[1].data_ptr() for b in bufs], stream)
+                    else:
+                        ctx.hist_device([p], s.data_ptr(), c.data_ptr(), stream)
                     torch.cuda.synchronize()
                     wall = time.perf_counter() - t0
                     kern = ctx.stats()["hist_ns"] * 1e-9
@@ -646,8 +666,9 @@ def bench_aux(args):
 def bench_batch(args):
     """256 independent 2D slices of ragged N (log-uniform in [2000, 40000]); per slice the
     longitudinal second-order SF over 48 log distance bins and the joint 48 x 64 (distance, du_L)
-    histogram.  Batched: two plans (one per operator) over all slices, executed as captured
-    hipGraphs; baseline: one binary64 device call per slice and operator (512 launches)."""
+    histogram.  Batched: one fused plan over all slices (both outputs from one evaluation of each
+    pair) executed as a captured hipGraph, beside two one-operator plans; baseline: one binary64
+    device call per slice and operator (512 launches)."""
     import numpy as np
     import torch
 
@@ -674,27 +695,55 @@ def bench_batch(args):
                                 dedges, vbins[k] if joint else None, prec) for k in range(S)]
 
     prec = args.precision
+    # the batched form: one fused plan (both outputs from one evaluation of each pair; the joint
+    # output takes each slice's own value edges) replayed as one hipGraph
+    outs = []
+    for op, jv in (((0, 2, 0), False), ((0, 1, 0), True)):
+        o = _native.SfOutput()
+        o.op, o.nl, o.nt = op
+        if jv:
+            o.n_val, o.val_edges = nv, None
+        outs.append(o)
     t0 = time.perf_counter()
+    planf = _native.Plan(ctx, probs((0, 1, 0), True, prec), outs)
+    t_build = time.perf_counter() - t0
     plan1 = _native.Plan(ctx, probs((0, 2, 0), False, prec))
     planj = _native.Plan(ctx, probs((0, 1, 0), True, prec))
-    t_build = time.perf_counter() - t0
 
     def step_batched():
+        planf.execute_multi([out1_s.data_ptr(), outj_s.data_ptr()], [out1_c.data_ptr(), outj_c.data_ptr()],
+                            stream, graph=True)
+        planf.sync()
+
+    def step_two_plans():
         plan1.execute(out1_s.data_ptr(), out1_c.data_ptr(), stream, graph=True)
         planj.execute(outj_s.data_ptr(), outj_c.data_ptr(), stream, graph=True)
         plan1.sync()
         planj.sync()
 
-    for _ in range(args.warmup):
-        step_batched()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step_batched()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    dt = timed(step_batched, args.steps)
     # Gpairs/s: N(N-1)/2 summed over the slices per wall second (each pair feeds both outputs)
     rate = pairs / dt / 1e9
+    dt_two = timed(step_two_plans, max(1, args.steps // 2))
+    # one execution of each form from zeroed outputs: the counts must agree
+    res = []
+    for fn in (step_batched, step_two_plans):
+        for t in (out1_s, out1_c, outj_s, outj_c):
+            t.zero_()
+        fn()
+        torch.cuda.synchronize()
+        res.append((out1_c.cpu().numpy().copy(), outj_c.cpu().numpy().copy()))
+    two_counts_equal = all(np.array_equal(a, b) for a, b in zip(res[0], res[1]))
 
     # sequential binary64 launches, one per slice and operator
     seq_s = torch.zeros(nd * nv, dtype=torch.float64, device=dev)
@@ -758,6 +807,9 @@ def bench_batch(args):
            "dtype": DTYPE[prec], "data": "synthetic (256 seeded 2D Fourier-mode fields)",
            "config": {"workload": "256 slices, N log-uniform [2000, 40000], L2 1D 48 bins + joint du_L 48x64",
                       "slices": S, "pairs_per_operator": pairs, "plan_build_s": round(t_build, 3)},
+           "two_plans_gpairs_s": round(pairs / dt_two / 1e9, 3),
+           "speedup_fused_vs_two_plans": round(dt_two / dt, 3),
+           "two_plans_counts_equal_fused": bool(two_counts_equal),
            "sequential_fp64_gpairs_s": round(rate_seq, 3),
            "batched_fp64_gpairs_s": round(rate_b64, 3),
            "speedup_vs_sequential_fp64": round(rate / rate_seq, 3),

@@ -176,7 +176,8 @@ int sf_hist_batch_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices
 
 /* Fused outputs: one evaluation of every pair feeds several operators of the same point fields.
  * The slices give the fields, geometry, distance bins, precision and semantics as for
- * sf_hist_batch_device (their op, nl, nt, n_val, val_edges and val_rule are ignored); outs[k]
+ * sf_hist_batch_device (their op, nl, nt and n_val are ignored, their val_edges and val_rule too
+ * unless a joint output asks for them); outs[k]
  * gives operator k and, for at most one of them, value bins (a joint histogram).  Output k of
  * slice s accumulates into d_sums[k] / d_counts[k] at s * ncell_k (ncell_k = n_dist, or
  * n_dist * n_val for the joint output).  Results equal n_out separate calls (counts bit-exact;
@@ -192,7 +193,8 @@ typedef struct {
   int32_t op, nl, nt;            /* operator (SF_OP_*, exponents)                                 */
   int32_t n_val;                 /* 0: 1D over the distance bins; else the joint histogram's value
                                     bins (n_val + 1 ascending edges in val_edges, host pointer)    */
-  const double *val_edges;
+  const double *val_edges;       /* NULL with n_val > 0: each slice's own val_edges and val_rule
+                                    (per-slice value bins; each must hold n_val + 1 edges)        */
   const sf_bin_rule *val_rule;   /* NULL or the value axis' LinearBinEdges rule                   */
 } sf_output;
 int sf_hist_multi_device(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t n_out,
@@ -245,6 +247,18 @@ int sf_plan_execute(sf_plan *plan, double *d_sums, uint64_t *d_counts, void *str
                     int32_t use_graph);
 int sf_plan_sync(sf_plan *plan, int32_t *redone);
 void sf_plan_destroy(sf_plan *plan);
+/* Fused plan: the outputs of sf_hist_multi_device (at most SF_MAX_FUSED 1D operators and one
+ * joint histogram) from one launch per execution, replayable as one hipGraph.  Binary32
+ * precisions only, and every output must fuse (FullVector{1} and {3} do not): otherwise
+ * SF_E_UNSUPPORTED (make one plan per output).  sf_plan_execute_multi takes one device sums /
+ * counts pointer per output (laid out as for sf_hist_multi_device); sf_plan_sync and
+ * sf_plan_destroy serve both kinds (after a re-bin queue overflow, sync redoes every output with
+ * the binary64 kernel).  The reference's nearest counterpart is its batch-leading loop, which
+ * shares a pair's geometry across fields (src/Calculations/batch_leading.jl:105-140). */
+int sf_plan_create_multi(sf_ctx *ctx, int32_t n_slices, const sf_problem *slices, int32_t n_out,
+                         const sf_output *outs, sf_plan **out);
+int sf_plan_execute_multi(sf_plan *plan, double *const *d_sums, uint64_t *const *d_counts, void *stream,
+                          int32_t use_graph);
 
 /* Diagnostics of the last successful call on this context (for tests and benchmarks):
  * out[0] = pairs visited, out[1] = pairs re-binned in binary64 by the mixed kernel,

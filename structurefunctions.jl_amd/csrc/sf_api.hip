@@ -1067,7 +1067,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   // more waves per workgroup then pay: flat 2D 595 against 492 Gpairs/s, sphere 644 against 529).
   // One-operator 1D kernels keep 256: 512 gained 5 % on flat 2D at N = 262144, nothing at 65536,
   // and lost on small slices (N = 4096: 51 against 88; the 256 ragged slices: 1019 against 1031)
-  int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : (pl.sp ? 512 : 256));
+  // Fused 1D outputs without a joint one: 512 threads (their histograms, 16 copies each, fill the
+  // LDS at two or three workgroups per CU, like the single pass's)
+  int block = t_block ? t_block : (J ? (pl.fx ? 768 : 1024) : ((pl.sp || pl.n1d) ? 512 : 256));
   // column records staged per LDS chunk; the fixed-point joint launch keeps two workgroups per CU
   const int tj = std::min(block, (pl.fx || strip) ? 128 : 256);
   // column records: variant 1 stages them through LDS, variant 2 reads them with scalar loads
@@ -1083,8 +1085,10 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   if (!J) {
     // per-lane-group copies of a 1D histogram (lanes l and l + copies share one): 16 measured
     // best on both binary32 1D kernels (flat 2D du_L^2 at N = 65536: 636 / 643 / 840 / 708 / 597
-    // Gpairs/s for 64 / 32 / 16 / 8 / 4 copies; sphere at N = 262144: 601 / 605 / 890 / 732 / 610)
-    copies = tn.copies ? tn.copies : 16;
+    // Gpairs/s for 64 / 32 / 16 / 8 / 4 copies; sphere at N = 262144: 601 / 605 / 890 / 732 / 610).
+    // A fused launch without a joint output does not use this histogram (its outputs have their
+    // own): one copy.
+    copies = pl.n1d ? 1 : (tn.copies ? tn.copies : 16);
     while (copies > 1 &&
            (size_t)smem_layout(p0.n_dist, 0, ncell_max, stage_bytes, copies, nmom, 0, 0, strip).total * 2 > ctx->max_smem)
       copies >>= 1;
@@ -1310,6 +1314,7 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   P.n1d = pl.n1d;
   P.mcopies = mcop;
   P.n_slices = pl.nout;
+  P.mab_a = P.mab_b = 0;
   for (int k = 0; k < pl.n1d; ++k) {
     P.mop[k] = multi->op[k];
     P.mnl[k] = multi->nl[k];
@@ -1317,10 +1322,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
     int a = 0, b = 0;
     if (!fused_basis_index(multi->op[k], multi->nl[k], multi->nt[k], &a, &b))
       return ctx->fail(SF_E_DEVICE, "internal: fused output %d is not a basis product", k);
-    P.ma[k] = a;
-    P.mb[k] = b;
-  }
-  for (int k = pl.n1d; k < kMaxFused; ++k) P.ma[k] = P.mb[k] = 0;  // unused outputs: 1 * 1
+    P.mab_a |= (uint32_t)a << (4 * k);
+    P.mab_b |= (uint32_t)b << (4 * k);
+  }  // unused outputs: fields 0, 1 * 1
   P.ja = P.jb = 0;
   if (pl.n1d && J && !fused_basis_index(p0.op, p0.nl, p0.nt, &P.ja, &P.jb))
     return ctx->fail(SF_E_DEVICE, "internal: the fused joint operator is not a basis product");
@@ -1554,103 +1558,134 @@ extern "C" int sf_hist_device(sf_ctx *ctx, const sf_problem *p, double *d_sums, 
 // ----------------------------------------------------------------------------------------------
 // fused outputs: one pair evaluation, several operators
 // ----------------------------------------------------------------------------------------------
-extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out,
-                                    const sf_output *outs, double *const *d_sums, uint64_t *const *d_counts,
-                                    void *stream) {
-  if (!ctx) return SF_E_ARG;
-  ctx->err.clear();
-  if (n_out < 1 || !outs || !d_sums || !d_counts) return ctx->fail(SF_E_ARG, "a fused call needs at least one output");
-  int joint = -1, n1 = 0, first1 = -1;
+// What a fused call does with its outputs: each output's own problems (validated as its own call
+// would be), the joint output fused into the launch (-1: none), the fused 1D outputs in launch
+// order, and the outputs that run as their own calls (binary64 precision: all of them; else those
+// whose binary32 value is not a product of two basis values, FullVector{1} and {3}).
+struct FusedSetup {
+  std::vector<std::vector<sf_problem>> per;
+  int joint = -1;
+  std::vector<int> ids;
+  std::vector<int> separate;
+  MultiSpec ms;
+  const std::vector<sf_problem> &main() const { return per[joint >= 0 ? joint : ids[0]]; }
+  bool launch() const { return joint >= 0 || !ids.empty(); }
+};
+
+int fused_setup(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out, const sf_output *outs,
+                FusedSetup &fs) {
+  if (n_out < 1 || !outs) return ctx->fail(SF_E_ARG, "a fused call needs at least one output");
+  int joint = -1, n1 = 0;
   for (int k = 0; k < n_out; ++k) {
-    if (!d_sums[k] || !d_counts[k]) return ctx->fail(SF_E_ARG, "output %d: null sums or counts", k);
     if (outs[k].op == SF_OP_SINGLE_PASS)
       return ctx->fail(SF_E_ARG, "output %d: the single pass is not a fused output (it is its own fused form)", k);
     if (outs[k].n_val > 0) {
       if (joint >= 0) return ctx->fail(SF_E_UNSUPPORTED, "a fused call takes at most one joint output");
       joint = k;
     } else {
-      if (first1 < 0) first1 = k;
       ++n1;
     }
   }
   if (n1 > SF_MAX_FUSED) return ctx->fail(SF_E_UNSUPPORTED, "a fused call takes at most %d 1D outputs", SF_MAX_FUSED);
-  // each output is validated as its own call would be
-  std::vector<std::vector<sf_problem>> per((size_t)n_out);
+  fs.per.assign((size_t)n_out, {});
   for (int k = 0; k < n_out; ++k) {
-    per[k].assign(sl, sl + std::max(n, 0));
-    for (sf_problem &q : per[k]) {
+    fs.per[k].assign(sl, sl + std::max(n, 0));
+    for (sf_problem &q : fs.per[k]) {
       q.op = outs[k].op;
       q.nl = outs[k].nl;
       q.nt = outs[k].nt;
       q.n_val = outs[k].n_val;
-      q.val_edges = outs[k].val_edges;
-      q.val_rule = outs[k].val_rule;
+      if (outs[k].n_val == 0 || outs[k].val_edges) {  // else each slice's own value edges
+        q.val_edges = outs[k].val_edges;
+        q.val_rule = outs[k].val_rule;
+      }
     }
-    const int rc = validate_batch(ctx, n, per[k].data(), false);
+    const int rc = validate_batch(ctx, n, fs.per[k].data(), false);
     if (rc) return rc;
   }
-  if (n == 0) return SF_OK;
-  HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  // binary64: one output after the other (the fused kernel computes in binary32)
-  if (sl[0].precision == SF_PREC_FP64) {
-    for (int k = 0; k < n_out; ++k) {
-      const int rc = sf_hist_batch_device(ctx, n, per[k].data(), d_sums[k], d_counts[k], st);
-      if (rc) return rc;
-    }
+  fs.joint = -1;
+  fs.ids.clear();
+  fs.separate.clear();
+  fs.ms = MultiSpec();
+  if (n > 0 && sl[0].precision == SF_PREC_FP64) {
+    for (int k = 0; k < n_out; ++k) fs.separate.push_back(k);
     return SF_OK;
   }
-  MultiSpec ms;
-  Plan &pl = ctx->fused;
-  // 1D outputs whose binary32 value is not a product of two basis values (FullVector{1}, {3}:
-  // a square root per pair) run as their own calls after the fused launch
-  std::vector<int> separate;
-  first1 = -1;
-  {
-    int a = 0, b = 0;
-    if (joint >= 0 && !fused_basis_index(outs[joint].op, outs[joint].nl, outs[joint].nt, &a, &b)) {
-      separate.push_back(joint);
-      joint = -1;
-    }
+  int a = 0, b = 0;
+  if (joint >= 0) {
+    if (fused_basis_index(outs[joint].op, outs[joint].nl, outs[joint].nt, &a, &b)) fs.joint = joint;
+    else fs.separate.push_back(joint);
   }
   for (int k = 0; k < n_out; ++k) {
-    if (k == joint || (outs[k].n_val > 0 && joint < 0)) continue;
-    int a = 0, b = 0;
+    if (k == joint) continue;
     if (!fused_basis_index(outs[k].op, outs[k].nl, outs[k].nt, &a, &b)) {
-      separate.push_back(k);
+      fs.separate.push_back(k);
       continue;
     }
-    if (first1 < 0) first1 = k;
-    ms.op[ms.n1d] = outs[k].op;
-    ms.nl[ms.n1d] = outs[k].nl;
-    ms.nt[ms.n1d] = outs[k].nt;
-    pl.mdst_s[ms.n1d] = d_sums[k];
-    pl.mdst_c[ms.n1d] = (unsigned long long *)d_counts[k];
-    ++ms.n1d;
+    fs.ms.op[fs.ms.n1d] = outs[k].op;
+    fs.ms.nl[fs.ms.n1d] = outs[k].nl;
+    fs.ms.nt[fs.ms.n1d] = outs[k].nt;
+    ++fs.ms.n1d;
+    fs.ids.push_back(k);
   }
-  for (int k : separate) {
-    const int rc = sf_hist_batch_device(ctx, n, per[k].data(), d_sums[k], d_counts[k], st);
-    if (rc) return rc;
-  }
-  if (joint < 0 && first1 < 0) return SF_OK;
-  // the plan's main problem: the joint output's (its value bins), or the first 1D output's,
-  // whose own histogram then stays unused
-  const std::vector<sf_problem> &main = per[joint >= 0 ? joint : first1];
-  std::vector<const double *> px(n), pu(n);
-  std::vector<double> xmax(n, 0.0);
+  return SF_OK;
+}
+
+// coordinate magnitudes of flat slices (the binary32 kernels' margins), read from the device
+int slice_xmax(sf_ctx *ctx, int n, const sf_problem *sl, hipStream_t st, double scale, std::vector<double> &xmax) {
+  xmax.assign((size_t)n, 0.0);
   for (int s = 0; s < n; ++s) {
-    px[s] = sl[s].x;
-    pu[s] = sl[s].u;
     if (sl[s].geometry == SF_GEOM_FLAT && sl[s].N > 0) {
       std::vector<double> hx((size_t)sl[s].W * sl[s].N);
       HIPCHK(hipMemcpyAsync(hx.data(), sl[s].x, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       double m = 0;
       for (double v : hx) m = std::max(m, fabs(v));
-      xmax[s] = m;
+      xmax[s] = scale * m;
     }
   }
-  int rc = plan_build(ctx, pl, n, main.data(), px.data(), pu.data(), xmax.data(), st, true, &ms);
+  return SF_OK;
+}
+
+extern "C" int sf_hist_multi_device(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out,
+                                    const sf_output *outs, double *const *d_sums, uint64_t *const *d_counts,
+                                    void *stream) {
+  if (!ctx) return SF_E_ARG;
+  ctx->err.clear();
+  if (n_out < 1 || !outs || !d_sums || !d_counts) return ctx->fail(SF_E_ARG, "a fused call needs at least one output");
+  for (int k = 0; k < n_out; ++k)
+    if (!d_sums[k] || !d_counts[k]) return ctx->fail(SF_E_ARG, "output %d: null sums or counts", k);
+  FusedSetup fs;
+  int rc = fused_setup(ctx, n, sl, n_out, outs, fs);
+  if (rc) return rc;
+  if (n == 0) return SF_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  for (int k : fs.separate) {
+    rc = sf_hist_batch_device(ctx, n, fs.per[k].data(), d_sums[k], d_counts[k], st);
+    if (rc) return rc;
+  }
+  if (!fs.launch()) return SF_OK;
+  const std::vector<int> &separate = fs.separate;
+  const std::vector<std::vector<sf_problem>> &per = fs.per;
+  const int joint = fs.joint;
+  Plan &pl = ctx->fused;
+  for (size_t j = 0; j < fs.ids.size(); ++j) {
+    pl.mdst_s[j] = d_sums[fs.ids[j]];
+    pl.mdst_c[j] = (unsigned long long *)d_counts[fs.ids[j]];
+  }
+  // the plan's main problem: the joint output's (its value bins), or the first 1D output's,
+  // whose own histogram then stays unused
+  const std::vector<sf_problem> &main = fs.main();
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax;
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+  }
+  rc = slice_xmax(ctx, n, sl, st, 1.0, xmax);
+  if (rc) return rc;
+  rc = plan_build(ctx, pl, n, main.data(), px.data(), pu.data(), xmax.data(), st, true, &fs.ms);
   if (rc) return rc;
   for (int k = 0; k < 8; ++k) ctx->stats[k] = 0;
   ctx->stats[0] = pl.total_pairs;
@@ -1709,8 +1744,10 @@ extern "C" int sf_hist_multi(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32
       q.nl = outs[0].nl;
       q.nt = outs[0].nt;
       q.n_val = outs[0].n_val;
-      q.val_edges = outs[0].val_edges;
-      q.val_rule = outs[0].val_rule;
+      if (outs[0].n_val == 0 || outs[0].val_edges) {
+        q.val_edges = outs[0].val_edges;
+        q.val_rule = outs[0].val_rule;
+      }
     }
     const int rc = validate_batch(ctx, n, chk.data(), true);
     if (rc) return rc;
@@ -1851,6 +1888,12 @@ struct sf_plan {
   hipStream_t last_stream = nullptr;
   double *last_s = nullptr;
   unsigned long long *last_c = nullptr;
+  // fused plan (sf_plan_create_multi): its outputs' problems and launch positions, and the last
+  // execution's destinations (for the binary64 redo after a re-bin queue overflow)
+  int n_out = 0;
+  FusedSetup fs;
+  std::vector<double *> last_ms;
+  std::vector<unsigned long long *> last_mc;
 };
 
 extern "C" int sf_plan_create(sf_ctx *ctx, int32_t n, const sf_problem *sl, sf_plan **out) {
@@ -1894,6 +1937,7 @@ extern "C" int sf_plan_execute(sf_plan *plan, double *d_sums, uint64_t *d_counts
                                int32_t use_graph) {
   if (!plan) return SF_E_ARG;
   sf_ctx *ctx = plan->ctx;
+  if (plan->n_out) return ctx->fail(SF_E_ARG, "a fused plan: use sf_plan_execute_multi");
   if (!d_sums || !d_counts) return ctx->fail(SF_E_ARG, "null sums or counts");
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
@@ -1931,6 +1975,99 @@ extern "C" int sf_plan_execute(sf_plan *plan, double *d_sums, uint64_t *d_counts
   return SF_OK;
 }
 
+extern "C" int sf_plan_create_multi(sf_ctx *ctx, int32_t n, const sf_problem *sl, int32_t n_out,
+                                    const sf_output *outs, sf_plan **out) {
+  if (!ctx || !out) return SF_E_ARG;
+  *out = nullptr;
+  ctx->err.clear();
+  if (n <= 0) return ctx->fail(SF_E_ARG, "a plan needs at least one slice");
+  sf_plan *plan = new sf_plan();
+  plan->ctx = ctx;
+  plan->n_out = n_out;
+  int rc = fused_setup(ctx, n, sl, n_out, outs, plan->fs);
+  if (!rc && !plan->fs.separate.empty())
+    rc = ctx->fail(SF_E_UNSUPPORTED,
+                   "a fused plan needs a binary32 precision and outputs that fuse (output %d does not: "
+                   "binary64, or FullVector{1}/{3}); make one plan per output instead",
+                   plan->fs.separate[0]);
+  if (rc) {
+    delete plan;
+    return rc;
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  std::vector<const double *> px(n), pu(n);
+  std::vector<double> xmax;
+  for (int s = 0; s < n; ++s) {
+    px[s] = sl[s].x;
+    pu[s] = sl[s].u;
+  }
+  // later contents must stay within 2x of the coordinate magnitude read now
+  rc = slice_xmax(ctx, n, sl, st, 2.0, xmax);
+  // no culling in plans: the positions behind x may change between executions
+  if (!rc) rc = plan_build(ctx, plan->pl, n, plan->fs.main().data(), px.data(), pu.data(), xmax.data(), st, false,
+                           &plan->fs.ms);
+  if (rc) {
+    plan->pl.release();
+    delete plan;
+    return rc;
+  }
+  *out = plan;
+  return SF_OK;
+}
+
+extern "C" int sf_plan_execute_multi(sf_plan *plan, double *const *d_sums, uint64_t *const *d_counts, void *stream,
+                                     int32_t use_graph) {
+  if (!plan) return SF_E_ARG;
+  sf_ctx *ctx = plan->ctx;
+  if (plan->n_out == 0) return ctx->fail(SF_E_ARG, "not a fused plan (use sf_plan_execute)");
+  if (!d_sums || !d_counts) return ctx->fail(SF_E_ARG, "null output arrays");
+  for (int k = 0; k < plan->n_out; ++k)
+    if (!d_sums[k] || !d_counts[k]) return ctx->fail(SF_E_ARG, "output %d: null sums or counts", k);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  Plan &pl = plan->pl;
+  const FusedSetup &fs = plan->fs;
+  bool moved = false;
+  for (size_t j = 0; j < fs.ids.size(); ++j) {
+    moved |= pl.mdst_s[j] != d_sums[fs.ids[j]] || pl.mdst_c[j] != (unsigned long long *)d_counts[fs.ids[j]];
+    pl.mdst_s[j] = d_sums[fs.ids[j]];
+    pl.mdst_c[j] = (unsigned long long *)d_counts[fs.ids[j]];
+  }
+  double *js = fs.joint >= 0 ? d_sums[fs.joint] : nullptr;
+  unsigned long long *jc = fs.joint >= 0 ? (unsigned long long *)d_counts[fs.joint] : nullptr;
+  plan->last_stream = st;
+  plan->last_ms.assign(d_sums, d_sums + plan->n_out);
+  plan->last_mc.assign((unsigned long long *const *)d_counts, (unsigned long long *const *)d_counts + plan->n_out);
+  if (!use_graph) return plan_launch(pl, true, js, jc, st, nullptr);
+  if (!pl.graph || moved || pl.g_sums != js || pl.g_counts != jc || pl.g_stream != st) {
+    if (pl.graph) {
+      HIPCHK(hipGraphExecDestroy(pl.graph));
+      pl.graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = plan_launch(pl, true, js, jc, st, nullptr);
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return ctx->fail(SF_E_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(&pl.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      pl.graph = nullptr;
+      return ctx->fail(SF_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    pl.g_sums = js;
+    pl.g_counts = jc;
+    pl.g_stream = st;
+  }
+  HIPCHK(hipGraphLaunch(pl.graph, st));
+  return SF_OK;
+}
+
 extern "C" int sf_plan_sync(sf_plan *plan, int32_t *redone) {
   if (!plan) return SF_E_ARG;
   sf_ctx *ctx = plan->ctx;
@@ -1942,7 +2079,19 @@ extern "C" int sf_plan_sync(sf_plan *plan, int32_t *redone) {
   if (!pl.mixed || pl.total_pairs == 0) return SF_OK;
   int flag = 0;
   HIPCHK(hipMemcpy(&flag, pl.flag.p, sizeof(int), hipMemcpyDeviceToHost));
-  if (flag) {
+  if (flag && plan->n_out) {
+    // a fused plan's last execute overflowed the re-bin queue and left every output untouched:
+    // each output with the binary64 kernel
+    for (int k = 0; k < plan->n_out; ++k) {
+      std::vector<sf_problem> q = plan->fs.per[k];
+      for (sf_problem &e : q) e.precision = SF_PREC_FP64;
+      const int rc = sf_hist_batch_device(ctx, (int)q.size(), q.data(), plan->last_ms[k],
+                                          (uint64_t *)plan->last_mc[k], plan->last_stream);
+      if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(plan->last_stream));
+    if (redone) *redone = 1;
+  } else if (flag) {
     // the last execute overflowed the re-bin queue and left the outputs untouched
     const int rc = plan_launch(pl, false, plan->last_s, plan->last_c, plan->last_stream, nullptr);
     if (rc) return rc;

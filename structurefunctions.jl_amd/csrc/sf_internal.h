@@ -141,9 +141,10 @@ struct Params {
   int32_t mcopies;         // lane-group copies of each fused 1D histogram (power of two)
   int32_t n_slices;
   int32_t mop[8], mnl[8], mnt[8];
-  // binary32 value of fused 1D output k: X[ma[k]] * X[mb[k]] over the per-pair basis
-  // X = {1, du_L, du_L^2, du_L^3, du_T, tn2, du_T^3, |du|^2, tn2 / (D - 1)} (fused_basis_index)
-  int32_t ma[8], mb[8];
+  // binary32 value of fused 1D output k: X[a_k] * X[b_k] over the per-pair basis
+  // X = {1, du_L, du_L^2, du_L^3, du_T, tn2, du_T^3, |du|^2, tn2 / (D - 1)} (fused_basis_index);
+  // a_k and b_k are the 4-bit fields k of mab_a and mab_b
+  uint32_t mab_a, mab_b;
   int32_t ja, jb;          // the joint operator's basis indices (fused plan with a joint output)
   double *msums;
   unsigned long long *mcounts;
@@ -238,6 +239,17 @@ inline constexpr int64_t kQueueMax = 1LL << 28;
 // velocity record in registers)
 inline constexpr int kStripMax = 4;
 
+// Fused 1D outputs: the kernel evaluates and accumulates a fixed number NF >= n1d of them (the
+// outputs past n1d are 1 * 1 into histograms that are never flushed), so its loop has no branch
+// per output.  NF is 1, 2, 4, 6 or 8; kernel class (OPK) 9 .. 13 respectively.
+inline __host__ __device__ int fused_pad(int n1d) {
+  return n1d <= 2 ? n1d : (n1d <= 4 ? 4 : (n1d <= 6 ? 6 : 8));
+}
+inline __host__ __device__ int fused_opk(int n1d) {
+  const int nf = fused_pad(n1d);
+  return nf == 1 ? 9 : (nf == 2 ? 10 : (nf == 4 ? 11 : (nf == 6 ? 12 : 13)));
+}
+
 // copies: 1D lane-group copies, or the joint histogram's copies (1 or 2); jlayout: joint layout
 inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int ncell_max,
                                                   int stage_bytes, int copies, int nmom, int fx,
@@ -254,9 +266,9 @@ inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int nce
   const int nz = fx ? kZeroBins * n_dist + kDummySlots : 0;
   L.zs = o; o += 8 * nz;
   L.zc = o; o += 4 * ((nz + 3) & ~3);
-  // fused 1D outputs: n1d histograms of n_dist x mcopies slots sharing one count per slot
+  // fused 1D outputs: fused_pad(n1d) histograms of n_dist x mcopies slots sharing one count per slot
   const int nm = n1d > 0 ? n_dist * mcopies + kDummySlots : 0;
-  L.ms = o; o += 8 * nm * n1d;
+  L.ms = o; o += 8 * nm * fused_pad(n1d);
   L.mc = o; o += 4 * ((nm + 3) & ~3);
   L.tab = o; o += 16 * ncell_max;
   L.dE = o; o += 8 * ((n_dist + 2) & ~1);

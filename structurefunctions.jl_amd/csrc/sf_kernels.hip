@@ -1010,7 +1010,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   // OPK 9, 10, 11: fused outputs -- the joint operator (if J) and P.n1d <= NF 1D operators per
   // pair, every value a product of two basis values (fused_basis_index)
   constexpr bool MULTI = (OPK >= 9);
-  constexpr int NF = OPK == 9 ? 2 : (OPK == 10 ? 4 : kMaxFused);
+  constexpr int NF = OPK == 9 ? 1 : (OPK == 10 ? 2 : (OPK == 11 ? 4 : (OPK == 12 ? 6 : kMaxFused)));
   constexpr bool needN2 = (OPK == 0) || MULTI || SP || SP2;
   const int nb = P.n_dist;
   const bool upper = !MASK || j > i;
@@ -1219,7 +1219,13 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     float mag = 0.f;
 #pragma unroll
     for (int k = 0; k < NF; ++k) {
-      mv[k] = X[P.ma[k]] * X[P.mb[k]];
+      // the indices are extracted here, by an opaque scalar bit-field extract: hoisted out of the
+      // loop, 2 NF indices would hold SGPRs of their own (at NF = 8 they spilled to VGPR lanes,
+      // read back with a v_readlane per lookup)
+      int ia, ib;
+      asm volatile("s_bfe_u32 %0, %1, %2" : "=s"(ia) : "s"(P.mab_a), "i"((4 * k) | (4 << 16)));
+      asm volatile("s_bfe_u32 %0, %1, %2" : "=s"(ib) : "s"(P.mab_b), "i"((4 * k) | (4 << 16)));
+      mv[k] = X[ia] * X[ib];
       mag += __builtin_fabsf(mv[k]);
     }
     amb = amb || !(mag < __builtin_inff());
@@ -1233,9 +1239,15 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const bool add1 = upper & !amb & db_ok;
     const int h1 = add1 ? (db - 1) * m.mcop + (int)(__lane_id() & (m.mcop - 1)) : m.mscr;
     atomicAdd(&m.mc[h1], 1u);
+    double *const mp = m.ms + h1;
 #pragma unroll
-    for (int k = 0; k < NF; ++k)
-      if (k < P.n1d) atomicAdd(&m.ms[k * m.mstr + h1], (double)mv[k]);
+    for (int k = 0; k < NF; ++k) {
+      // output k's histogram is k mstr slots on: the offset is formed here by an opaque scalar
+      // multiply, not hoisted into an SGPR per output (those spilled to VGPR lanes)
+      int off = 0;
+      if (k > 0) asm volatile("s_mul_i32 %0, %1, %2" : "=s"(off) : "s"(m.mstr), "i"(k));
+      atomicAdd(mp + off, (double)mv[k]);
+    }
   }
   int h;
   // 24-bit multiplies (full rate): every operand of a slot that is used is a small bin index;
@@ -1432,9 +1444,11 @@ DEV void run_chunk32(const Params &P, const Slice &s, const Smem &m, const float
 // the compiler turns into scalar loads (SGPR operands, scalar cache) -- no LDS staging, no
 // barriers inside an item.  Otherwise columns are staged through LDS in chunks of P.tj.
 // Fixed-point SLOAD kernels are held to 6 waves per SIMD (80 VGPRs): two 768-thread workgroups
-// per CU, which their small LDS footprint allows; the few values this spills are per-item ones.
+// per CU, which their small LDS footprint allows; the few values this spills are per-item ones
+// (the fused ones spill a few SGPR-held values of the pair loop too, and still gain: the 256
+// ragged 2D slices' du_L^2 and joint du_L from one launch, 73.7 -> 57.6 ms).
 template <int G, int D, bool J, int OPK, bool SLOAD, bool FX>
-__global__ void __launch_bounds__(1024, (FX && SLOAD && OPK < 9) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
+__global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
   const Smem m = carve(smem_raw, P, P.stage_bytes);
@@ -2048,6 +2062,8 @@ static const void *hist32_fn(int opk) {
   if (opk == 9) return (const void *)&hist32<G, D, J, 9, true, FX>;
   if (opk == 10) return (const void *)&hist32<G, D, J, 10, true, FX>;
   if (opk == 11) return (const void *)&hist32<G, D, J, 11, true, FX>;
+  if (opk == 12) return (const void *)&hist32<G, D, J, 12, true, FX>;
+  if (opk == 13) return (const void *)&hist32<G, D, J, 13, true, FX>;
   if constexpr (!J) {
     if (opk == 3) return (const void *)&hist32<G, D, false, 3, SL, false>;
     if constexpr (SL) {  // 1D, scalar-loaded columns (the default): L^3, T, T^2, T^3
@@ -2096,7 +2112,7 @@ static const void *pick_hist(int G, int D, bool J, bool mixed, int opk, bool sl,
 }
 
 static int opk_of(const Params &P) {
-  if (P.n1d > 0) return P.n1d <= 2 ? 9 : (P.n1d <= 4 ? 10 : 11);
+  if (P.n1d > 0) return fused_opk(P.n1d);
   if (sp2d(P.op, P.n_val)) return 4;
   if (P.nmom > 1) return 3;
   if (P.op == 0 && P.nl == 1 && P.nt == 0) return 1;

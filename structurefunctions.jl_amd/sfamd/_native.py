@@ -25,7 +25,8 @@ EXPORTED = ("sf_create", "sf_destroy", "sf_last_error", "sf_hist", "sf_hist_batc
             "sf_hist_device", "sf_hist_batch_device", "sf_last_stats", "sf_device_info", "sf_check",
             "sf_eval_math",
             "sf_set_tuning", "sf_plan_create", "sf_plan_execute", "sf_plan_sync", "sf_plan_destroy",
-            "sf_autotune", "sf_hist_batch_device_noalloc", "sf_hist_multi_device", "sf_hist_multi")
+            "sf_autotune", "sf_hist_batch_device_noalloc", "sf_hist_multi_device", "sf_hist_multi",
+            "sf_plan_create_multi", "sf_plan_execute_multi")
 SF_MAX_FUSED = 8
 
 
@@ -141,6 +142,11 @@ def lib():
     L.sf_hist_multi.argtypes = [vp, i32, ctypes.POINTER(SfProblem), i32, ctypes.POINTER(SfOutput),
                                 ctypes.POINTER(vp), ctypes.POINTER(vp)]
     L.sf_hist_multi.restype = ctypes.c_int
+    L.sf_plan_create_multi.argtypes = [vp, i32, ctypes.POINTER(SfProblem), i32, ctypes.POINTER(SfOutput),
+                                       ctypes.POINTER(vp)]
+    L.sf_plan_create_multi.restype = ctypes.c_int
+    L.sf_plan_execute_multi.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp, i32]
+    L.sf_plan_execute_multi.restype = ctypes.c_int
     _lib = L
     return L
 
@@ -268,12 +274,25 @@ class Context:
 class Plan:
     """sf_plan: a batch built once and executed many times (optionally as a hipGraph)."""
 
-    def __init__(self, ctx: Context, problems):
+    def __init__(self, ctx: Context, problems, outputs=None):
+        """outputs: None for a one-operator plan, else a list of SfOutput (a fused plan)."""
         self.ctx = ctx
         self._problems = (SfProblem * len(problems))(*problems)
+        self.n_out = 0 if outputs is None else len(outputs)
         h = ctypes.c_void_p()
-        ctx.check(lib().sf_plan_create(ctx.handle, len(problems), self._problems, ctypes.byref(h)))
+        if outputs is None:
+            ctx.check(lib().sf_plan_create(ctx.handle, len(problems), self._problems, ctypes.byref(h)))
+        else:
+            self._outs = (SfOutput * len(outputs))(*outputs)
+            ctx.check(lib().sf_plan_create_multi(ctx.handle, len(problems), self._problems, len(outputs),
+                                                 self._outs, ctypes.byref(h)))
         self._h = h
+
+    def execute_multi(self, d_sums_ptrs, d_counts_ptrs, stream_ptr: int = 0, graph: bool = True):
+        ps = (ctypes.c_void_p * self.n_out)(*[int(a) for a in d_sums_ptrs])
+        pc = (ctypes.c_void_p * self.n_out)(*[int(a) for a in d_counts_ptrs])
+        self.ctx.check(lib().sf_plan_execute_multi(self._h, ps, pc, ctypes.c_void_p(stream_ptr) if stream_ptr else None,
+                                                   1 if graph else 0))
 
     def execute(self, d_sums_ptr: int, d_counts_ptr: int, stream_ptr: int = 0, graph: bool = True):
         self.ctx.check(lib().sf_plan_execute(self._h, ctypes.c_void_p(d_sums_ptr), ctypes.c_void_p(d_counts_ptr),

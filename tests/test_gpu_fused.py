@@ -181,3 +181,87 @@ def test_fused_rejects(ctx):
         fused(ctx, [(x, u)], dedges, ["joint", "joint"], "flat", "mixed", ve)
     with pytest.raises(sfamd.ArgumentError):
         fused(ctx, [(x, u)], dedges, [(_native.OP_SINGLE_PASS, 0, 0)], "flat", "mixed")
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fixed"])
+def test_fused_plan_graph_replays(ctx, prec):
+    """A fused plan (du_L^2 1D and the joint du_L histogram of ragged 2D slices, the batched
+    configuration's outputs) replayed as one hipGraph after the fields were rewritten in place and
+    after unrelated eager work: every replay equals the fused eager call on the same data, and the
+    first the oracle."""
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(17)
+    sizes = [900, 1, 1500, 64]
+    e = np.geomspace(1e-3, 1.5, 25)
+    ve = np.linspace(-3, 3, 33)
+    nd, nv = len(e) - 1, len(ve) - 1
+    xs = [torch.zeros((n, 2), dtype=torch.float64, device=dev) for n in sizes]
+    us = [torch.zeros((n, 2), dtype=torch.float64, device=dev) for n in sizes]
+
+    def problems():
+        out = []
+        for x, u in zip(xs, us):
+            p = _native.SfProblem()
+            p.D, p.W, p.N = 2, 2, x.shape[0]
+            p.x, p.u = x.data_ptr(), u.data_ptr()
+            p.geometry, p.radius, p.op, p.nl, p.nt = 0, 1.0, 0, 2, 0
+            p.n_dist, p.dist_edges = nd, e.ctypes.data
+            p.precision, p.count_bits = PREC[prec], 64
+            out.append(p)
+        return out
+
+    keep = []
+    outs = _outputs([(0, 2, 0), "joint"], ve, keep)
+    plan = _native.Plan(ctx, problems(), outs)
+    stream = torch.cuda.current_stream().cuda_stream
+    B = len(sizes)
+    bufs = [(torch.zeros(B * nd, dtype=torch.float64, device=dev), torch.zeros(B * nd, dtype=torch.int64, device=dev)),
+            (torch.zeros(B * nd * nv, dtype=torch.float64, device=dev),
+             torch.zeros(B * nd * nv, dtype=torch.int64, device=dev))]
+    ebufs = [(torch.zeros_like(a), torch.zeros_like(b)) for a, b in bufs]
+    for rep in range(3):
+        hx = [rng.uniform(0, 1, (n, 2)) for n in sizes]
+        hu = [rng.normal(size=(n, 2)) for n in sizes]
+        for x, u, a, b in zip(xs, us, hx, hu):
+            x.copy_(torch.from_numpy(a))
+            u.copy_(torch.from_numpy(b))
+        junk = torch.randn(1 << 20, device=dev).sum()
+        for a, b in bufs + ebufs:
+            a.zero_()
+            b.zero_()
+        plan.execute_multi([a.data_ptr() for a, _ in bufs], [b.data_ptr() for _, b in bufs], stream, graph=True)
+        assert not plan.sync()
+        ctx.hist_multi_device(problems(), outs, [a.data_ptr() for a, _ in ebufs], [b.data_ptr() for _, b in ebufs],
+                              stream)
+        torch.cuda.synchronize()
+        for (gs, gc), (es, ec) in zip(bufs, ebufs):
+            assert np.array_equal(gc.cpu().numpy(), ec.cpu().numpy()), rep
+            np.testing.assert_allclose(gs.cpu().numpy(), es.cpu().numpy(), rtol=1e-12, atol=1e-12)
+        if rep == 0:
+            for k, n in enumerate(sizes):
+                if n < 2:
+                    continue
+                for o, (op, vv, nc) in enumerate([((0, 2, 0), None, nd), (JOINT, ve, nd * nv)]):
+                    gs = bufs[o][0].cpu().numpy()[k * nc:(k + 1) * nc]
+                    gc = bufs[o][1].cpu().numpy()[k * nc:(k + 1) * nc].astype(np.uint64)
+                    _check(gs, gc, hx[k].T, hu[k].T, e, op, "flat", prec, vv, f"replay slice {k} output {o}")
+        float(junk)
+    plan.close()
+
+
+def test_fused_plan_rejects(ctx):
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    x = torch.rand((100, 2), dtype=torch.float64, device=dev)
+    e = np.geomspace(1e-3, 1.5, 9)
+    for prec, ops in [("fp64", [(0, 2, 0)]), ("mixed", [(0, 2, 0), (5, 1, 0)])]:
+        p = _native.SfProblem()
+        p.D, p.W, p.N = 2, 2, 100
+        p.x, p.u = x.data_ptr(), x.data_ptr()
+        p.geometry, p.radius, p.op, p.nl, p.nt = 0, 1.0, 0, 2, 0
+        p.n_dist, p.dist_edges = len(e) - 1, e.ctypes.data
+        p.precision, p.count_bits = PREC[prec], 64
+        keep = []
+        with pytest.raises(sfamd.ArgumentError):
+            _native.Plan(ctx, [p], _outputs(ops, None, keep))
