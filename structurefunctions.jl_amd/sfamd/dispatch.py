@@ -34,7 +34,9 @@ _CONTEXTS: dict = {}
 class MI355XBackend:
     """Execution backend running the pair histogram on one MI355X (gfx950) GPU.
 
-    ``precision="fp64"`` is the drop-in binary64 kernel (counts bit-identical to SerialBackend);
+    ``precision="fp64"`` is the drop-in binary64 kernel (counts bit-identical to the oracle's
+    restatement of SerialBackend; against Julia itself only up to the ulp-level differences of its
+    sincosd/atan and LogBinEdges' log-grid lookup, which no test here can pin);
     ``precision="mixed"`` runs the pair arithmetic in binary32, re-bins every pair whose distance
     or value lies within the binary32 error bound of a bin edge in binary64 (counts still
     bit-identical), and accumulates in binary64.  ``precision="mixed_fixed"`` is "mixed" with
