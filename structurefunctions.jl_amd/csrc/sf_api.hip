@@ -1078,9 +1078,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   // fixed-point launch needs); 1D histograms with 16 copies, N = 262144, sphere 974 vs 908 and
   // flat 2D 1158 vs 958 (no LDS stage: more workgroups per CU)
   const bool sload = mixed && !strip && t_variant != 1;
-  const int rec_bytes = mixed ? F32 * 4 : S64 * 8;
-  // a B-strip stages every field's column records
-  const int stage_bytes = sload ? 0 : tj * rec_bytes * (strip ? strip : 1);
+  // a B-strip stages every field's column records; the binary64 kernel stages none (scalar loads)
+  const int stage_bytes = (sload || !mixed) ? 0 : tj * F32 * 4 * (strip ? strip : 1);
   int copies = 1;
   if (!J) {
     // per-lane-group copies of a 1D histogram (lanes l and l + copies share one): 16 measured
@@ -1111,7 +1110,8 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   const SmemLayout L0 = smem_layout(p0.n_dist, p0.n_val, ncell_max, stage_bytes, copies, nmom, pl.fx, 0, strip,
                                     jlayout, pl.n1d, mcop);
   // the binary64 kernel keeps one copy of a joint histogram
-  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, tj * S64 * 8, J ? 1 : copies, nmom, 0, 0, 0,
+  // (the binary64 kernel reads its column records with scalar loads: no LDS stage)
+  const SmemLayout L64 = smem_layout(p0.n_dist, p0.n_val, ncell_max, 0, J ? 1 : copies, nmom, 0, 0, 0,
                                      jlayout);
   if ((size_t)std::max(L0.total, L64.total) > ctx->max_smem)
     return ctx->fail(SF_E_UNSUPPORTED,
@@ -1357,7 +1357,7 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
     HIPCHK(launch_zero(pl.mcounts.p, sizeof(unsigned long long) * mcells, st));
   }
   const bool use_sload = use_mixed && pl.sload;
-  const int stage = use_sload ? 0 : pl.tj * (use_mixed ? pl.F32 * 4 : pl.S64 * 8);
+  const int stage = (use_sload || !use_mixed) ? 0 : pl.tj * pl.F32 * 4;
   Params P = pl.P;
   P.stage_bytes = stage;
   if (!use_mixed) {  // the binary64 kernel keeps binary64 sums and 32-bit counts, no queue

@@ -764,33 +764,48 @@ DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32
 // hist64: binary64 exact pair arithmetic for every pair
 // ---------------------------------------------------------------------------------------------
 template <int G, int D, bool J, int OPK, bool MASK>
-DEV void run_chunk64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i,
-                     int jb, int cnt) {
-  constexpr int S = Rec<G, D>::d64;
-  const double *st = (const double *)m.stage;
-  for (int jj = 0; jj < cnt; ++jj) {
-    const int j = jb + jj;
-    if (MASK && j <= i) continue;
-    double rj[S];
-    const double2 *st2 = (const double2 *)st;
-#pragma unroll
-    for (int k = 0; k < S / 2; ++k) {
-      const double2 q = st2[jj * (S / 2) + k];
-      rj[2 * k] = q.x;
-      rj[2 * k + 1] = q.y;
-    }
-    double v[2];
-    int c;
-    if constexpr (G == kSphere)
-      c = exact_cell<G, D, J, OPK, true>(P, s, ri, rj, P.cells + s.tab_off, m.dE, m.vE, v, m.lohi, m.blo);
-    else
-      c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
-    if (c >= 0) {
-      const int h = hslot(m, c % P.n_dist, c / P.n_dist);
-      if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
-      else hist_add(m, h, v[0]);
-    }
+DEV void pair64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j,
+                const double *rj) {
+  if (MASK && j <= i) return;
+  double v[2];
+  int c;
+  if constexpr (G == kSphere)
+    c = exact_cell<G, D, J, OPK, true>(P, s, ri, rj, P.cells + s.tab_off, m.dE, m.vE, v, m.lohi, m.blo);
+  else
+    c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
+  if (c >= 0) {
+    const int h = hslot(m, c % P.n_dist, c / P.n_dist);
+    if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
+    else hist_add(m, h, v[0]);
   }
+}
+
+template <int S>
+DEV void load_rec64(const double *__restrict__ p, double *r) {
+#pragma unroll
+  for (int k = 0; k < S; ++k) r[k] = p[k];
+}
+
+// Columns j0 .. j1 - 1 of one row: the column records are read straight from global memory at
+// wave-uniform addresses (scalar loads into SGPRs: no LDS stage, no barrier per chunk), two
+// record buffers alternating so that the next column's record is requested before the current
+// pair is evaluated.  The request past the last column reads at most one record beyond the
+// slice: the next slice's first record, or the record array's tail pad.
+template <int G, int D, bool J, int OPK, bool MASK>
+DEV void run_cols64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
+                    const double *__restrict__ rec) {
+  constexpr int S = Rec<G, D>::d64;
+  const double *col = rec + (int64_t)s.pt_off * S;
+  double ra[S], rb[S];
+  load_rec64<S>(col + (int64_t)j0 * S, ra);
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
+    load_rec64<S>(col + (int64_t)(j + 1) * S, rb);
+    pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra);
+    load_rec64<S>(col + (int64_t)(j + 2) * S, ra);
+    pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb);
+  }
+  if (j < j1) pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra);
 }
 
 template <int G, int D, bool J, int OPK>
@@ -825,6 +840,7 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
         load_tables(m, P, s, G == kSphere);
       }
       since_flush = 0;
+      __syncthreads();  // tables and cleared histogram visible before the item's pairs
     }
     since_flush += work;
     const int i = w.i0 + (int)threadIdx.x;
@@ -832,17 +848,9 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
     double ri[S];
 #pragma unroll
     for (int k = 0; k < S; ++k) ri[k] = iv ? rec[(s.pt_off + i) * S + k] : 0.0;
-    for (int jb = w.j0; jb < w.j1; jb += P.tj) {
-      const int cnt = min(P.tj, w.j1 - jb);
-      __syncthreads();
-      const double2 *src = (const double2 *)(rec + (s.pt_off + jb) * S);
-      double2 *dst = (double2 *)m.stage;
-      for (int k = threadIdx.x; k < cnt * (S / 2); k += B) dst[k] = src[k];
-      __syncthreads();
-      if (iv) {
-        if (jb < w.i0 + B) run_chunk64<G, D, J, OPK, true>(P, s, m, ri, i, jb, cnt);
-        else run_chunk64<G, D, J, OPK, false>(P, s, m, ri, i, jb, cnt);
-      }
+    if (iv) {
+      if (w.j0 < w.i0 + B) run_cols64<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
+      else run_cols64<G, D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
     }
   }
   __syncthreads();
