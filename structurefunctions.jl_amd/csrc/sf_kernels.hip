@@ -220,23 +220,22 @@ DEV int dist_bin64(double key, const Cell *tab, const double *E, int nE, const S
 // digitize(x, ::LinearBinEdges) = searchsortedfirst(v, x) - 1 (BinEdges.jl:114-131), or inside
 // InfPaddedBinEdges (:306-316): the histogram bin, valid 1 .. nb (0 and nb + 1 are out of range).
 // muladd is fused.  A NaN query (the reference's floor(Int, NaN) throws) is out of range.
+// Branch-free: every case is evaluated and the answer selected.  The reference's Int64 index
+// clamp(floor(t) + 1, 1, n) is formed in binary64, which is exact wherever it is used (x in
+// (f, l], so |t| <= n + 1 < 2^31; the host keeps n below 2^31); elsewhere NaN and infinities are
+// clamped away (max / min return the number) before the conversion.
 DEV int lin_rule_bin(const LinRule &r, double x, int nb) {
-  if (x != x) return nb + 1;
-  if (r.padded && x <= -__builtin_inf()) return 0;
-  int64_t k;
-  if (x <= r.f) {
-    k = 1;
-  } else if (x > r.l) {
-    k = r.n + 1;
-  } else {
-    const double c = -r.f * r.inv;  // a separate product, as in the reference
-    const double t = __builtin_fma(x, r.inv, c);
-    int64_t idx = (int64_t)__builtin_floor(t) + 1;
-    idx = idx < 1 ? 1 : (idx > r.n ? r.n : idx);
-    const double u = __builtin_fma((double)(idx - 1), r.step, r.f);
-    k = u < x ? idx + 1 : idx;
-  }
-  return (int)(r.padded ? k : k - 1);
+  const double c = -r.f * r.inv;  // a separate product, as in the reference
+  const double t = __builtin_fma(x, r.inv, c);
+  const double n = (double)r.n;
+  const double fi = __builtin_fmin(__builtin_fmax(__builtin_floor(t) + 1.0, 1.0), n);
+  const double u = __builtin_fma(fi - 1.0, r.step, r.f);
+  int k = (int)fi + (u < x ? 1 : 0);
+  k = (x <= r.f) ? 1 : k;
+  k = (x > r.l) ? (int)r.n + 1 : k;
+  k = r.padded ? k : k - 1;
+  k = (r.padded && x <= -__builtin_inf()) ? 0 : k;
+  return (x != x) ? nb + 1 : k;
 }
 
 // searchsortedfirst(value_edges, v) - 1 with Julia isless semantics
@@ -299,7 +298,7 @@ DEV int cell32(float key, const Slice &s);
 template <int G, int D, bool J, int OPK = 0, bool CERT = false>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
                    const Cell *tab, const double *dE, const double *vE, double *val,
-                   const float2 *lohi = nullptr, const int *blo_tab = nullptr) {
+                   const float2 *lohi = nullptr, const int *blo_tab = nullptr, int *bins = nullptr) {
   const int nEd = P.n_dist + 1;
   int db;
   double v;
@@ -318,6 +317,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       const double r = __builtin_sqrt(r2);
       db = s.drule.on ? lin_rule_bin(s.drule, r, P.n_dist) : dist_bin64(r, tab, dE, nEd, s);
       if (db < 1 || db > P.n_dist) return -1;
+      if (bins) bins[0] = db - 1;
       double rh[D];
 #pragma unroll
       for (int k = 0; k < D; ++k) rh[k] = dx[k] / r;
@@ -342,6 +342,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       *val = v;
       if (!J) return db - 1;
       const int vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, vE, P.n_val + 1, s);
+      if (bins) bins[1] = vb - 1;
       if (vb < 1 || vb > P.n_val) return -1;
       return (db - 1) + P.n_dist * (vb - 1);
     }
@@ -349,6 +350,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     // BinEdges.jl:506,516) the linear rule on sqrt(r^2)
     db = s.drule.on ? lin_rule_bin(s.drule, __builtin_sqrt(r2), P.n_dist) : dist_bin64(r2, tab, dE, nEd, s);
     if (db < 1 || db > P.n_dist) return -1;
+    if (bins) bins[0] = db - 1;
     if constexpr (OPK == 3 || OPK == 4) {
       const double inv_r = 1.0 / __builtin_sqrt(r2);
       const double duL = sdot<D>(du, dx) * inv_r;
@@ -389,6 +391,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
     if (exact_dist)
       db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist) : dist_bin64(f.distance, tab, dE, nEd, s);
     if (!f.ok || db < 1 || db > P.n_dist) return -1;
+    if (bins) bins[0] = db - 1;
     double du[3];
     sphere_delta64<D>(f, A, B, du);
     const double rh[3] = {1.0, 0.0, 0.0};
@@ -409,6 +412,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
   *val = v;
   if (!J) return db - 1;
   const int vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, vE, P.n_val + 1, s);
+  if (bins) bins[1] = vb - 1;
   if (vb < 1 || vb > P.n_val) return -1;
   return (db - 1) + P.n_dist * (vb - 1);
 }
@@ -768,17 +772,19 @@ template <int G, int D, bool J, int OPK, bool MASK>
 DEV void pair64(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j,
                 const double *rj) {
   if (MASK && j <= i) return;
-  double v[2];
+  double v[2] = {0.0, 0.0};
+  int bins[2] = {0, 0};
   int c;
   if constexpr (G == kSphere)
-    c = exact_cell<G, D, J, OPK, true>(P, s, ri, rj, P.cells + s.tab_off, m.dE, m.vE, v, m.lohi, m.blo);
+    c = exact_cell<G, D, J, OPK, true>(P, s, ri, rj, P.cells + s.tab_off, m.dE, m.vE, v, m.lohi, m.blo, bins);
   else
-    c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v);
-  if (c >= 0) {
-    const int h = hslot(m, c % P.n_dist, c / P.n_dist);
-    if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
-    else hist_add(m, h, v[0]);
-  }
+    c = exact_cell<G, D, J, OPK>(P, s, ri, rj, m.tab, m.dE, m.vE, v, nullptr, nullptr, bins);
+  // the slot from the bins (no division of c), and a pair that is not counted goes to the lane's
+  // scratch slot (never flushed): the update is unconditional, the loop body has no branch here
+  int h = J ? bins[1] * m.jv + bins[0] * m.jd + m.coff : bins[0] * m.copies + (int)(__lane_id() & (m.copies - 1));
+  h = c >= 0 ? h : m.scr;
+  if constexpr (OPK == 3) hist_add_sp(m, h, v[0], v[1]);
+  else hist_add(m, h, v[0]);
 }
 
 template <int S>
@@ -807,6 +813,89 @@ DEV void run_cols64(const Params &P, const Slice &s, const Smem &m, const double
     pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb);
   }
   if (j < j1) pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra);
+}
+
+// Sphere: the same pairs as pair64 / exact_cell (the certified chord^2 distance bin, the
+// reference's operation sequence for every value), arranged like the binary32 kernel's pair: the
+// previous pair's histogram update is deferred until this pair's table read is in flight (LDS
+// operations complete in order: the wait for the read would otherwise wait for the atomics), the
+// next column's record is requested only after that wait (a scalar load in flight would be
+// waited for with it), and the body has no branch but the rare one that computes the central
+// angle of a pair whose chord^2 is in a cell's ambiguity interval.  A pair that is not counted
+// updates the lane's scratch slot.
+struct Pending64 {
+  int h;
+  double v, w;  // value (single pass: du_L) and |du|^2 (single pass)
+};
+
+template <int D, bool J, int OPK, bool MASK>
+DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A, int i, int j,
+                 const double *B, double *rn, const double *next, Pending64 &pa) {
+  constexpr int S = Rec<kSphere, D>::d64;
+  SphereFrame f = sphere_frame64_noangle(A, B);
+  const float key = (float)sdot<3>(f.d, f.d);
+  const int c = cell32(key, s);
+  const float2 lh = m.lohi[c];
+  const int blo = m.blo[c];
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
+  else hist_add(m, pa.h, pa.v);
+  asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
+  const bool exact_dist = (key >= lh.x) & (key <= lh.y);
+  int db = (key > lh.y) ? blo + 1 : blo;
+  load_rec64<S>(next, rn);
+  __builtin_amdgcn_sched_barrier(0);
+  double du[3];
+  sphere_delta64<D>(f, A, B, du);
+  const double rh[3] = {1.0, 0.0, 0.0};
+  double v, w = 0.0;
+  if constexpr (OPK == 3) {
+    v = sdot<D>(du, rh);
+    w = sdot<D>(du, du);
+  } else if constexpr (OPK == 4) {
+    v = sp_invariant(s.plane, sdot<D>(du, rh), sdot<D>(du, du));
+  } else if constexpr (OPK == 1) {
+    v = 1.0 * sdot<D>(du, rh);
+  } else if constexpr (OPK == 2) {
+    const double L = sdot<D>(du, rh);
+    v = 1.0 * (L * L);
+  } else {
+    v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+  }
+  int vb = 1;
+  if constexpr (J) vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, m.vE, P.n_val + 1, s);
+  if (exact_dist) {
+    f.distance = sphere_distance64(A, B, f.d, P.radius);
+    db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist)
+                    : dist_bin64(f.distance, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+  }
+  const bool add = (!MASK || j > i) & f.ok & ((unsigned)(db - 1) < (unsigned)P.n_dist) &
+                   (!J || (unsigned)(vb - 1) < (unsigned)P.n_val);
+  const int h = J ? (vb - 1) * m.jv + (db - 1) * m.jd + m.coff
+                  : (db - 1) * m.copies + (int)(__lane_id() & (m.copies - 1));
+  pa.h = add ? h : m.scr;
+  pa.v = v;
+  pa.w = w;
+}
+
+// Columns j0 .. j1 - 1 of one row with pair64s; two record buffers alternate (the request past
+// the last column reads at most one record beyond the slice, as in run_cols64).
+template <int D, bool J, int OPK, bool MASK>
+DEV void run_cols64s(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
+                     const double *__restrict__ rec) {
+  constexpr int S = Rec<kSphere, D>::d64;
+  const double *col = rec + (int64_t)s.pt_off * S;
+  double ra[S], rb[S];
+  load_rec64<S>(col + (int64_t)j0 * S, ra);
+  Pending64 pa{m.scr, 0.0, 0.0};
+  int j = j0;
+  for (; j + 1 < j1; j += 2) {
+    pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+    pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb, ra, col + (int64_t)(j + 2) * S, pa);
+  }
+  if (j < j1) pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+  if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
+  else hist_add(m, pa.h, pa.v);
 }
 
 template <int G, int D, bool J, int OPK>
@@ -850,8 +939,13 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
 #pragma unroll
     for (int k = 0; k < S; ++k) ri[k] = iv ? rec[(s.pt_off + i) * S + k] : 0.0;
     if (iv) {
-      if (w.j0 < w.i0 + B) run_cols64<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
-      else run_cols64<G, D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
+      if constexpr (G == kSphere) {
+        if (w.j0 < w.i0 + B) run_cols64s<D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
+        else run_cols64s<D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
+      } else {
+        if (w.j0 < w.i0 + B) run_cols64<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
+        else run_cols64<G, D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
+      }
     }
   }
   __syncthreads();
