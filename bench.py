@@ -48,6 +48,9 @@ TABLE_FLOPS = {("cfg1", "L2"): 28,
                # fused outputs (one pair evaluation, sf_hist_multi_device), SURVEY-style counts:
                # flat 2D all six projected operators 40; sphere four 1D + joint du_L 76
                ("cfg2", "fused L1 T1 L2 T2 L3 T3"): 40, ("cfg4", "fused L2 T2 L3 T3 + joint L1"): 76}
+# (configuration, operator) rows also timed with the binary64 drop-in kernel (against the FP64 peak)
+FP64_ROWS = {("cfg1", "L2"), ("cfg2", "L2"), ("cfg2", "T3"), ("cfg3", "joint L1"), ("cfg4", "L2"),
+             ("cfg4", "joint L1")}
 FUSED = {"fused L1 T1 L2 T2 L3 T3": ["L1", "T1", "L2", "T2", "L3", "T3"],
          "fused L2 T2 L3 T3 + joint L1": ["L2", "T2", "L3", "T3", "joint L1"]}
 OPS = {"L1": (0, 1, 0), "T1": (0, 0, 1), "L2": (0, 2, 0), "T2": (0, 0, 2), "L3": (0, 3, 0),
@@ -500,7 +503,9 @@ def bench_table(args):
     binary32 kernel's time from the library's HIP events (best of --steps calls after one warm-up)
     and the whole call's wall time; achieved = pairs x FLOP-equivalents per pair / kernel time,
     against the FP32 peak recomputed from the device's compute units and clock (256 FLOP per clock
-    per CU) and against the published 157.3 TFLOP/s.  One JSON line."""
+    per CU) and against the published 157.3 TFLOP/s.  The FP64_ROWS cases are also timed with the
+    binary64 drop-in kernel, against the FP64 peak (78.6 TFLOP/s published, half the device's
+    FP32 figure).  One JSON line."""
     import numpy as np
     import torch
 
@@ -546,6 +551,8 @@ def bench_table(args):
                 bufs.append((torch.zeros(n_o, dtype=torch.float64, device=dev),
                              torch.zeros(n_o, dtype=torch.int64, device=dev)))
             precs = ["mixed_fixed", "mixed"] if joint else ["mixed"]
+            if (cfg, name) in FP64_ROWS:
+                precs = precs + ["fp64"]
             for prec in precs:
                 p = _device_problem(_native, dx.data_ptr(), du.data_ptr(), N, D, W, geom,
                                     OPS[fused[0]] if fused else OPS[name], de, None if fused else vedges, prec)
@@ -566,17 +573,20 @@ def bench_table(args):
                         best_w = wall if best_w is None else min(best_w, wall)
                 f = TABLE_FLOPS[(cfg, name)]
                 ach = pairs * f / best_k / 1e12
+                # binary64 rows: the same FLOP-equivalents against the FP64 peak (half the FP32 one)
+                peak_pub, peak_d = (PEAK_FP64_TFLOPS, peak_dev / 2) if prec == "fp64" else (PEAK_FP32_TFLOPS, peak_dev)
                 rows.append({"config": cfg, "workload": desc, "operator": name, "precision": prec,
                              "pairs": pairs, "flops_per_pair": f,
                              "kernel_ms": round(best_k * 1e3, 3), "call_ms": round(best_w * 1e3, 3),
                              "kernel_gpairs_s": round(pairs / best_k / 1e9, 2),
                              "call_gpairs_s": round(pairs / best_w / 1e9, 2),
-                             "achieved_tflops": round(ach, 2),
-                             "frac_published": round(ach / PEAK_FP32_TFLOPS, 4),
-                             "frac_device": round(ach / peak_dev, 4)})
+                             "achieved_tflops": round(ach, 2), "peak_tflops": peak_pub,
+                             "frac_published": round(ach / peak_pub, 4),
+                             "frac_device": round(ach / peak_d, 4)})
                 print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
         del dx, du
-    out = {"metric": "per-configuration roofline of the binary32 kernels (kernel-time FLOP-equivalents)",
+    out = {"metric": "per-configuration roofline of the binary32 kernels and the binary64 drop-in "
+                     "(kernel-time FLOP-equivalents)",
            "unit": "TFLOP/s", "peak_published": PEAK_FP32_TFLOPS, "peak_device": round(peak_dev, 2),
            "device": info, "steps": args.steps, "data": "synthetic (seeded configurations 1-4)",
            "rows": rows}

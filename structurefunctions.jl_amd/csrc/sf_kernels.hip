@@ -862,12 +862,31 @@ DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A
   } else {
     v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
   }
+  // value bin (1-based).  Uniform value edges (a LinearBinEdges rule or an equally spaced edge
+  // vector): read from t = (v - v0) / w in binary32, certified like the binary32 kernel's (the
+  // host slack covers the edges' distance from v0 + k w, and the rule's decision points' distance
+  // from the edges), with twice its relative term for the rounding of v - v0 to binary32.  Only a
+  // value within that margin of a decision point (or non-finite) takes the exact rule or search.
   int vb = 1;
-  if constexpr (J) vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, m.vE, P.n_val + 1, s);
-  if (exact_dist) {
-    f.distance = sphere_distance64(A, B, f.d, P.radius);
-    db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist)
-                    : dist_bin64(f.distance, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+  bool amb_v = false;
+  if constexpr (J) {
+    if (s.v_uniform) {
+      const float t = (float)(v - s.v0) * s.vinvf;
+      const float eps = __builtin_fmaf(9.6e-7f, __builtin_fabsf(t), s.vslack);
+      const float gh = __builtin_floorf(t + eps);
+      amb_v = !(t - eps >= gh);
+      vb = amb_v ? 0 : (int)gh + 1;  // |t| < 2^21 when certain: the conversion is exact
+    } else {
+      vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, m.vE, P.n_val + 1, s);
+    }
+  }
+  if (exact_dist | amb_v) {
+    if (exact_dist) {
+      f.distance = sphere_distance64(A, B, f.d, P.radius);
+      db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist)
+                      : dist_bin64(f.distance, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+    }
+    if (J && amb_v) vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, m.vE, P.n_val + 1, s);
   }
   const bool add = (!MASK || j > i) & f.ok & ((unsigned)(db - 1) < (unsigned)P.n_dist) &
                    (!J || (unsigned)(vb - 1) < (unsigned)P.n_val);

@@ -83,3 +83,30 @@ def test_sphere_linear_distance_axis(prec):
     r = sfamd.calculate_structure_function(sfamd.L1SF, x, u, de, ve, np.uint64, backend=be,
                                            distance_metric=sfamd.Haversine(1.0))
     _check(r, s, c, a, f"sphere joint {prec}", 1e-6 if prec != "fp64" else 1e-12)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("rule", [True, False])
+def test_sphere_values_on_value_edges(prec, rule):
+    """Values that sit exactly on a value edge: a third of the points carry zero velocity, so the
+    pairs among them have du_L = +-0, on the edge at 0 of range(-2, 2, length=41) (as the rule, or
+    as the equally spaced edge vector); quarter-unit velocities along the equator put many more
+    values on edges.  The kernels read uniform value bins from a binary32 t = (v - v0) / w and
+    must hand every such value to the exact rule or search: counts bit-identical to the oracle."""
+    rng = np.random.default_rng(17)
+    n = 600
+    lon = np.concatenate([rng.uniform(-180, 180, n - 200), np.arange(200) * 1.5 - 150.0])
+    lat = np.concatenate([np.degrees(np.arcsin(rng.uniform(-1, 1, n - 200))), np.zeros(200)])
+    x = np.stack([lon, lat])
+    u = rng.integers(-8, 9, size=(2, n)) * 0.25
+    u[:, : n // 3] = 0.0
+    u[1, n - 200:] = 0.0
+    lin = bins.LinearBinEdges(-2.0, 2.0, 41)
+    ve = lin if rule else lin.edges()
+    assert 0.0 in np.asarray(lin.edges())
+    de = np.array([0.0, 0.01, 0.1, 0.5, 1.0, 2.0, np.pi])
+    be = sfamd.MI355XBackend(0, prec)
+    s, c, a = _oracle.hist(x, u, de, op=(0, 1, 0), geometry="deg", vedges=ve)
+    r = sfamd.calculate_structure_function(sfamd.L1SF, x, u, de, ve, np.uint64, backend=be,
+                                           distance_metric=sfamd.Haversine(1.0))
+    _check(r, s, c, a, f"sphere values on edges {prec} rule={rule}", 1e-6 if prec != "fp64" else 1e-12)
