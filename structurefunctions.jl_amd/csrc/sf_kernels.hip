@@ -815,6 +815,17 @@ DEV void run_cols64(const Params &P, const Slice &s, const Smem &m, const double
   if (j < j1) pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra);
 }
 
+// Row of the work item's tile that this thread owns: lane l of wave w takes row l (B / 64) + w,
+// so that a wave's 64 rows lie B / 64 points apart rather than side by side.  Neighbouring points
+// of a smooth field have nearly the same distance to a column and nearly the same velocity, so a
+// wave of adjacent rows sends many lanes to one histogram slot in the same instruction, and LDS
+// atomics to one address serialise.  Every tile still covers rows i0 .. i0 + B - 1.
+DEV int tile_row() {
+  const int B = blockDim.x;
+  const int t = threadIdx.x;
+  return (B & 63) ? t : (t & 63) * (B >> 6) + (t >> 6);
+}
+
 // Sphere: the same pairs as pair64 / exact_cell (the certified chord^2 distance bin, the
 // reference's operation sequence for every value), arranged like the binary32 kernel's pair: the
 // previous pair's histogram update is deferred until this pair's table read is in flight (LDS
@@ -952,7 +963,7 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
       __syncthreads();  // tables and cleared histogram visible before the item's pairs
     }
     since_flush += work;
-    const int i = w.i0 + (int)threadIdx.x;
+    const int i = w.i0 + tile_row();
     const bool iv = i < s.N;
     double ri[S];
 #pragma unroll
@@ -1606,7 +1617,7 @@ __global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, 
       since_flush = 0;
     }
     since_flush += work;
-    const int i = w.i0 + (int)threadIdx.x;
+    const int i = w.i0 + tile_row();
     const bool iv = i < s.N;
     float ri[F];
 #pragma unroll
