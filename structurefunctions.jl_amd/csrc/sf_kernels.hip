@@ -826,25 +826,42 @@ DEV int tile_row() {
   return (B & 63) ? t : (t & 63) * (B >> 6) + (t >> 6);
 }
 
-// Sphere: the same pairs as pair64 / exact_cell (the certified chord^2 distance bin, the
-// reference's operation sequence for every value), arranged like the binary32 kernel's pair: the
+// Sphere, and flat point fields (not the batch-leading pairs): the same pairs as pair64 /
+// exact_cell, with the reference's operation sequence for every value, arranged like the binary32
+// kernel's pair.  The distance bin is read from the binary32 key table (chord^2 or r^2: its
+// margins, 2^-20 relative and more, exceed the binary32 rounding of the binary64 key); the
 // previous pair's histogram update is deferred until this pair's table read is in flight (LDS
 // operations complete in order: the wait for the read would otherwise wait for the atomics), the
 // next column's record is requested only after that wait (a scalar load in flight would be
-// waited for with it), and the body has no branch but the rare one that computes the central
-// angle of a pair whose chord^2 is in a cell's ambiguity interval.  A pair that is not counted
-// updates the lane's scratch slot.
+// waited for with it), and the body has no branch but the rare one that bins a pair whose key is
+// in a cell's ambiguity interval (sphere: with the central angle) or whose value is within its
+// margin of a value decision point, by the exact rules.  Flat keys that are not above 2^-100
+// (coincident points, NaN) also take the exact rules.  A pair that is not counted updates the
+// lane's scratch slot.
 struct Pending64 {
   int h;
   double v, w;  // value (single pass: du_L) and |du|^2 (single pass)
 };
 
-template <int D, bool J, int OPK, bool MASK>
-DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A, int i, int j,
+template <int G, int D, bool J, int OPK, bool MASK>
+DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A, int i, int j,
                  const double *B, double *rn, const double *next, Pending64 &pa) {
-  constexpr int S = Rec<kSphere, D>::d64;
-  SphereFrame f = sphere_frame64_noangle(A, B);
-  const float key = (float)sdot<3>(f.d, f.d);
+  constexpr int S = Rec<G, D>::d64;
+  SphereFrame f;
+  double dx[D], du[3], r2 = 0.0;
+  float key;
+  if constexpr (G == kSphere) {
+    f = sphere_frame64_noangle(A, B);
+    key = (float)sdot<3>(f.d, f.d);
+  } else {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      dx[k] = B[k] - A[k];
+      du[k] = B[D + k] - A[D + k];
+    }
+    r2 = sdot<D>(dx, dx);
+    key = (float)r2;
+  }
   const int c = cell32(key, s);
   const float2 lh = m.lohi[c];
   const int blo = m.blo[c];
@@ -852,26 +869,52 @@ DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
   asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
-  const bool exact_dist = (key >= lh.x) & (key <= lh.y);
+  bool exact_dist = (key >= lh.x) & (key <= lh.y);
+  if constexpr (G == kFlat) exact_dist = exact_dist | !(key > 0x1p-100f);
   int db = (key > lh.y) ? blo + 1 : blo;
   load_rec64<S>(next, rn);
   __builtin_amdgcn_sched_barrier(0);
-  double du[3];
-  sphere_delta64<D>(f, A, B, du);
-  const double rh[3] = {1.0, 0.0, 0.0};
   double v, w = 0.0;
-  if constexpr (OPK == 3) {
-    v = sdot<D>(du, rh);
-    w = sdot<D>(du, du);
-  } else if constexpr (OPK == 4) {
-    v = sp_invariant(s.plane, sdot<D>(du, rh), sdot<D>(du, du));
-  } else if constexpr (OPK == 1) {
-    v = 1.0 * sdot<D>(du, rh);
-  } else if constexpr (OPK == 2) {
-    const double L = sdot<D>(du, rh);
-    v = 1.0 * (L * L);
+  if constexpr (G == kFlat) {
+    // exact_cell's flat point-field values (sf_raw64 and its specialisations)
+    if constexpr (OPK == 3 || OPK == 4) {
+      const double inv_r = 1.0 / __builtin_sqrt(r2);
+      const double duL = sdot<D>(du, dx) * inv_r;
+      const double dn2 = sdot<D>(du, du);
+      if constexpr (OPK == 3) {
+        v = duL;
+        w = dn2;
+      } else {
+        v = sp_invariant(s.plane, duL, dn2);
+      }
+    } else if constexpr (OPK == 2) {
+      const double p = sdot<D>(du, dx);
+      v = p * p / r2;
+    } else if constexpr (OPK == 1) {
+      const double sq = __builtin_sqrt(r2);
+      double rh[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) rh[k] = dx[k] / sq;
+      v = 1.0 * sdot<D>(du, rh);
+    } else {
+      v = sf_raw64<D>(P.op, P.nl, P.nt, du, dx, r2);
+    }
   } else {
-    v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+    sphere_delta64<D>(f, A, B, du);
+    const double rh[3] = {1.0, 0.0, 0.0};
+    if constexpr (OPK == 3) {
+      v = sdot<D>(du, rh);
+      w = sdot<D>(du, du);
+    } else if constexpr (OPK == 4) {
+      v = sp_invariant(s.plane, sdot<D>(du, rh), sdot<D>(du, du));
+    } else if constexpr (OPK == 1) {
+      v = 1.0 * sdot<D>(du, rh);
+    } else if constexpr (OPK == 2) {
+      const double L = sdot<D>(du, rh);
+      v = 1.0 * (L * L);
+    } else {
+      v = op_apply64<D>(P.op, P.nl, P.nt, du, rh);
+    }
   }
   // value bin (1-based).  Uniform value edges (a LinearBinEdges rule or an equally spaced edge
   // vector): read from t = (v - v0) / w in binary32, certified like the binary32 kernel's (the
@@ -893,13 +936,20 @@ DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A
   }
   if (exact_dist | amb_v) {
     if (exact_dist) {
-      f.distance = sphere_distance64(A, B, f.d, P.radius);
-      db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist)
-                      : dist_bin64(f.distance, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+      if constexpr (G == kSphere) {
+        f.distance = sphere_distance64(A, B, f.d, P.radius);
+        db = s.drule.on ? lin_rule_bin(s.drule, f.distance, P.n_dist)
+                        : dist_bin64(f.distance, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+      } else {
+        // squared-distance plan, or (SquaredLinearPlan) the linear rule on sqrt(r^2)
+        db = s.drule.on ? lin_rule_bin(s.drule, __builtin_sqrt(r2), P.n_dist)
+                        : dist_bin64(r2, P.cells + s.tab_off, m.dE, P.n_dist + 1, s);
+      }
     }
     if (J && amb_v) vb = s.vrule.on ? lin_rule_bin(s.vrule, v, P.n_val) : val_bin64(v, m.vE, P.n_val + 1, s);
   }
-  const bool add = (!MASK || j > i) & f.ok & ((unsigned)(db - 1) < (unsigned)P.n_dist) &
+  const bool ok = (G == kSphere) ? f.ok : true;
+  const bool add = (!MASK || j > i) & ok & ((unsigned)(db - 1) < (unsigned)P.n_dist) &
                    (!J || (unsigned)(vb - 1) < (unsigned)P.n_val);
   const int h = J ? (vb - 1) * m.jv + (db - 1) * m.jd + m.coff
                   : (db - 1) * m.copies + (int)(__lane_id() & (m.copies - 1));
@@ -908,22 +958,22 @@ DEV void pair64s(const Params &P, const Slice &s, const Smem &m, const double *A
   pa.w = w;
 }
 
-// Columns j0 .. j1 - 1 of one row with pair64s; two record buffers alternate (the request past
+// Columns j0 .. j1 - 1 of one row with pair64p; two record buffers alternate (the request past
 // the last column reads at most one record beyond the slice, as in run_cols64).
-template <int D, bool J, int OPK, bool MASK>
-DEV void run_cols64s(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
+template <int G, int D, bool J, int OPK, bool MASK>
+DEV void run_cols64p(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
                      const double *__restrict__ rec) {
-  constexpr int S = Rec<kSphere, D>::d64;
+  constexpr int S = Rec<G, D>::d64;
   const double *col = rec + (int64_t)s.pt_off * S;
   double ra[S], rb[S];
   load_rec64<S>(col + (int64_t)j0 * S, ra);
   Pending64 pa{m.scr, 0.0, 0.0};
   int j = j0;
   for (; j + 1 < j1; j += 2) {
-    pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
-    pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb, ra, col + (int64_t)(j + 2) * S, pa);
+    pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+    pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb, ra, col + (int64_t)(j + 2) * S, pa);
   }
-  if (j < j1) pair64s<D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+  if (j < j1) pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
 }
@@ -955,9 +1005,9 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
       if (w.slice != cur) {
         cur = w.slice;
         s = P.slices[cur];
-        // sphere: the chord^2 table of certified distance bins (the binary64 key table is read
-        // from global memory by the rare pairs that need it)
-        load_tables(m, P, s, G == kSphere);
+        // sphere and flat point fields: the binary32 key table of certified distance bins (the
+        // binary64 key table is read from global memory by the rare pairs that need it)
+        load_tables(m, P, s, G == kSphere || !P.bl);
       }
       since_flush = 0;
       __syncthreads();  // tables and cleared histogram visible before the item's pairs
@@ -969,9 +1019,9 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
 #pragma unroll
     for (int k = 0; k < S; ++k) ri[k] = iv ? rec[(s.pt_off + i) * S + k] : 0.0;
     if (iv) {
-      if constexpr (G == kSphere) {
-        if (w.j0 < w.i0 + B) run_cols64s<D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
-        else run_cols64s<D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
+      if (G == kSphere || !P.bl) {
+        if (w.j0 < w.i0 + B) run_cols64p<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
+        else run_cols64p<G, D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
       } else {
         if (w.j0 < w.i0 + B) run_cols64<G, D, J, OPK, true>(P, s, m, ri, i, w.j0, w.j1, rec);
         else run_cols64<G, D, J, OPK, false>(P, s, m, ri, i, w.j0, w.j1, rec);
