@@ -815,15 +815,20 @@ DEV void run_cols64(const Params &P, const Slice &s, const Smem &m, const double
   if (j < j1) pair64<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra);
 }
 
-// Row of the work item's tile that this thread owns: lane l of wave w takes row l (B / 64) + w,
-// so that a wave's 64 rows lie B / 64 points apart rather than side by side.  Neighbouring points
-// of a smooth field have nearly the same distance to a column and nearly the same velocity, so a
-// wave of adjacent rows sends many lanes to one histogram slot in the same instruction, and LDS
-// atomics to one address serialise.  Every tile still covers rows i0 .. i0 + B - 1.
+// Row of the work item's tile that this thread owns.  Joint histograms: lane l of wave w takes
+// row l (B / 64) + w, so that a wave's 64 rows lie B / 64 points apart rather than side by side.
+// Neighbouring points of a smooth field have nearly the same distance to a column and nearly the
+// same velocity, so a wave of adjacent rows sends many lanes to one joint slot in the same
+// instruction, and LDS atomics to one address serialise (binary64 spherical joint kernel: 192 ->
+// 162 ms at N = 262144).  1D histograms keep adjacent rows: their lane-group copies already keep
+// the atomics apart, and adjacent rows read the same cells of the key table (a broadcast), where
+// spread rows read scattered ones (4-5 % slower on the sphere).  Every tile still covers rows
+// i0 .. i0 + B - 1.
+template <bool J>
 DEV int tile_row() {
   const int B = blockDim.x;
   const int t = threadIdx.x;
-  return (B & 63) ? t : (t & 63) * (B >> 6) + (t >> 6);
+  return (!J || (B & 63)) ? t : (t & 63) * (B >> 6) + (t >> 6);
 }
 
 // Sphere, and flat point fields (not the batch-leading pairs): the same pairs as pair64 /
@@ -1013,7 +1018,7 @@ __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restric
       __syncthreads();  // tables and cleared histogram visible before the item's pairs
     }
     since_flush += work;
-    const int i = w.i0 + tile_row();
+    const int i = w.i0 + tile_row<J>();
     const bool iv = i < s.N;
     double ri[S];
 #pragma unroll
@@ -1667,7 +1672,7 @@ __global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, 
       since_flush = 0;
     }
     since_flush += work;
-    const int i = w.i0 + tile_row();
+    const int i = w.i0 + tile_row<J>();
     const bool iv = i < s.N;
     float ri[F];
 #pragma unroll
