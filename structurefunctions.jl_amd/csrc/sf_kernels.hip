@@ -5,6 +5,8 @@
 //   prep32<G,D>      per-point binary32 records for the mixed kernel
 //   hist64<G,D,J>    persistent upper-triangle tile kernel, binary64 pair arithmetic written in
 //                    the reference's operation order: counts bit-identical to SerialBackend
+//                    (bins certified from the exact key and value, the reference's rules where a
+//                    key or value is within the margin of a decision point)
 //   hist32<G,D,J>    same schedule, binary32 pair arithmetic; a pair whose distance key or value
 //                    lies within the binary32 error bound of a bin edge (or is degenerate) is
 //                    appended to a queue instead of being binned
@@ -12,9 +14,10 @@
 //   merge            adds the internal accumulators into the caller's outputs
 //
 // Schedule (both hist kernels): a work item is (slice, i-tile of BLOCK rows, column range
-// [j0, j1)).  Each thread owns one row i and keeps its record in registers; the block stages
-// BLOCK column records at a time in LDS and every wave reads them as broadcasts.  Pairs are
-// (i, j) with j > i.  Workgroups are persistent and pop items from a global counter; each keeps
+// [j0, j1)).  Each thread owns one row i and keeps its record in registers; the column records
+// are read at wave-uniform addresses, by scalar loads into SGPRs or (some binary32 variants)
+// staged BLOCK at a time in LDS, so every wave reads them as broadcasts.  Pairs are (i, j) with
+// j > i.  Workgroups are persistent and pop items from a global counter; each keeps
 // the current slice's histogram in LDS (binary64 sums, 32-bit counts) and flushes it to the
 // accumulators with global atomics when it changes slice, before its 32-bit counts could wrap,
 // and when it runs out of items.
