@@ -877,9 +877,10 @@ DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
   asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
-  bool exact_dist = (key >= lh.x) & (key <= lh.y);
+  const bool above = key > lh.y;
+  bool exact_dist = (key >= lh.x) & !above;
   if constexpr (G == kFlat) exact_dist = exact_dist | !(key > 0x1p-100f);
-  int db = (key > lh.y) ? blo + 1 : blo;
+  int db = above ? blo + 1 : blo;
   load_rec64<S>(next, rn);
   __builtin_amdgcn_sched_barrier(0);
   double v, w = 0.0;
@@ -1055,10 +1056,11 @@ DEV float close_key(const Slice &s) { return __builtin_fmaxf(1e-30f, 0x1p12f * s
 
 DEV int cell32(float key, const Slice &s) {
   const int bits = __float_as_int(key);
-  int c = (bits >> (23 - s.M32)) - s.base32;
-  c = c < 0 ? 0 : c;
-  c = c >= s.ncell32 ? s.ncell32 - 1 : c;
-  return c;
+  const int c = (bits >> (23 - s.M32)) - s.base32;
+  // clamp to [0, ncell32 - 1] in one instruction (the compiler forms max + min)
+  int r;
+  asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(c), "s"(s.ncell32 - 1));
+  return r;
 }
 
 
@@ -1243,8 +1245,10 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // operations): the next column's record is requested only after this point, because a
     // scalar load still in flight would make the wait for the table read wait for it as well
     asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
-    amb_d = (key >= lh.x) & (key <= lh.y);
-    db = (key > lh.y) ? blo + 1 : blo;
+    // one compare with hi serves both (a NaN key fails key >= lo either way)
+    const bool above = key > lh.y;
+    amb_d = (key >= lh.x) & !above;
+    db = above ? blo + 1 : blo;
   }
   // scalar-load variant: the request is unconditional (past the last column it reads the record
   // array's one-record tail pad) and pinned here, so that the scheduler does not sink it towards
