@@ -597,7 +597,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   s.v_uniform = 0;
   s.v_negzero = 0;
   s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0; s.vabs = 0;
-  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.plane = 0; s.vsmall = 0; s.gz = 0;
+  s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.fxc1 = 0; s.fxc0 = 0; s.plane = 0; s.vsmall = 0; s.gz = 0;
   s.v_log = 0; s.vk0 = 0; s.vl0f = 0; s.vlinvf = 0;
   if (p.n_val > 0) {
     const int nv = p.n_val;
@@ -624,6 +624,8 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
           s.v_fx = 1;
           s.vwf = (float)w;
           s.vsf = kFxScale / s.vwf;
+          s.fxc1 = (float)((double)s.vwf * (double)s.vsf);
+          s.fxc0 = (float)(8388608.0 + (double)kFxBias - (double)s.v0f * (double)s.vsf);
           // a certified bin g of a value with |v| < w/4 is within half a bin of [-w/4, w/4] (the
           // binary32 value's error is below half a bin, or the pair is re-binned), so g - gz is
           // 1 .. 3 with gz = floor((-w/4 - v0)/w) - 2; the kZeroBins = 5 cells gz .. gz + 4 leave

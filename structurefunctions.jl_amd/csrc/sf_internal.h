@@ -74,8 +74,12 @@ struct Slice {
   float vabs;         // binary32 value-axis absolute slack of searched edges (LinearBinEdges rule)
   int32_t v_fx;       // value edges uniform to 1/64 of a bin: fixed-point joint accumulation allowed
   // fixed-point joint accumulation: a value v of value bin g (0-based) is stored as the integer
-  // q = rint((v - a_g) * vsf + kFxBias) with a_g = fmaf(g, vwf, v0f) (binary32)
+  // part q of Q = fmaf(v, vsf, c_g) - 2^23, c_g = fmaf(-g, fxc1, fxc0) (binary32; fxc1 = vwf vsf,
+  // fxc0 = 2^23 + kFxBias - v0f vsf, so that Q ~ (v - (v0 + g w)) vsf + kFxBias); the flush
+  // recovers sum v = (sum q + n (2^23 - c_g)) / vsf with the same c_g, exact but for the rounding
+  // of each Q to an integer (half a unit, 2^-23 bin widths)
   float vwf, vsf;
+  float fxc1, fxc0;
   // ... except values with |v| < vsmall (a quarter bin width), whose fixed-point error would not be
   // small relative to the value: they go to binary64 cells of the value bins gz .. gz + kZeroBins - 1
   float vsmall;

@@ -658,7 +658,7 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
   // output cell c of the slice is out_off + c, or every sixth cell for a 2D single-pass plane
   const int64_t ostride = sp2d(P.op, P.n_val) ? kSpRows : 1;
   if (m.fx) {
-    // fixed-point joint cells: n = word >> 44, sum v = n a_g + (sum q - n kFxBias) / vsf
+    // fixed-point joint cells: n = word >> 44, sum v = (sum q + n (2^23 - c_g)) / vsf
     for (int c = threadIdx.x; c < m.ncell_h; c += blockDim.x) {
       const int db0 = c % m.nd, vb0 = c / m.nd;
       const int h0 = vb0 * m.jv + db0 * m.jd;
@@ -674,8 +674,10 @@ DEV void flush_hist(const Smem &m, const Params &P, const Slice &s) {
         sq += wd - nk * kFxUnit;
       }
       if (n) {
-        const double a = (double)__builtin_fmaf((float)vb0, s.vwf, s.v0f);
-        const double v = (double)n * a + ((double)sq - (double)n * (double)kFxBias) / (double)s.vsf;
+        // c_g exactly as the pair loop formed it; every term below is exact in binary64 but the
+        // division
+        const float cg = __builtin_fmaf(-(float)vb0, s.fxc1, s.fxc0);
+        const double v = ((double)sq + (double)n * (8388608.0 - (double)cg)) / (double)s.vsf;
         const int64_t o = s.out_off + ostride * c;
         atomicAdd(&P.counts[o], n);
         atomicAdd(&P.sums[o], v);
@@ -1071,15 +1073,15 @@ DEV int cell32(float key, const Slice &s) {
 template <bool VU>
 DEV int val_bin32(float v, float dva, float dvb, const Slice &s, const float *E32, int nE, bool &amb,
                   float &gf) {
-  // a non-finite value is undecidable here; the lookup below still runs, its bin is ignored
-  amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   if constexpr (VU) {  // uniform value edges (s.v_uniform; chosen once per column range)
     const float t = (v - s.v0f) * s.vinvf;
     // error of t: value bound (1/w > 0), host slack (edge non-uniformity, rounding of v0 and
     // 1/w), and the rounding of the subtraction, product and the two offsets below (a few ulp of
     // |t|).  Undecidable iff an integer lies in (t - eps, t + eps].  Far outside the edges
     // (|t| >= 2^21) eps exceeds 1, so such values are undecidable too (and re-binned); the
-    // float-to-int conversion saturates, so no clamp is needed.
+    // float-to-int conversion saturates, so no clamp is needed.  A non-finite v is undecidable
+    // without a test of its own: t = +-Inf makes eps infinite and t - eps or t + eps NaN, and a
+    // NaN fails the comparison below.
     const float eps = __builtin_fmaf(dva, dvb, __builtin_fmaf(4.8e-7f, __builtin_fabsf(t), s.vslack));
     // floor(t - eps) != floor(t + eps) exactly when t - eps < floor(t + eps) (NaN: undecidable)
     const float gh = __builtin_floorf(t + eps);
@@ -1087,6 +1089,8 @@ DEV int val_bin32(float v, float dva, float dvb, const Slice &s, const float *E3
     gf = gh;
     return (int)gh;
   }
+  // a non-finite value is undecidable here; the lookup below still runs, its bin is ignored
+  amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
   int lo = 0;
   if (s.v_log) {
     // log-spaced edges (positive from edge vk0 on): the first edge >= v guessed from log2(v),
@@ -1397,11 +1401,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     vb0 = val_bin32<VU>(v, dva, ampv, s, m.vE32, P.n_val + 1, amb, gf);
     vb_ok = (unsigned)vb0 < (unsigned)P.n_val;
     if constexpr (FX) {
-      // offset within the value bin in units of 2^-22 bin widths, biased, rounded to nearest
-      // kFxBias + 2^23 puts the result in [2^23, 2^24), where binary32 has unit spacing: the
-      // fma rounds to the nearest integer and the bits are kFxBase + q (no conversion needed)
-      const float a = __builtin_fmaf(gf, s.vwf, s.v0f);
-      pa.q = __float_as_uint(__builtin_fmaf(v - a, s.vsf, kFxBias + 8388608.0f));
+      // offset within the value bin in units of 2^-22 bin widths, biased, rounded to nearest:
+      // v vsf + c_g with c_g ~ 2^23 + kFxBias - a_g vsf (one fma, exact before its rounding)
+      // lies in [2^23, 2^24), where binary32 has unit spacing: the fma rounds to the nearest
+      // integer and the bits are kFxBase + q (no conversion needed)
+      pa.q = __float_as_uint(__builtin_fmaf(v, s.vsf, __builtin_fmaf(-gf, s.fxc1, s.fxc0)));
     }
   } else if constexpr (!SP) {
     amb = amb || !(__builtin_fabsf(v) < __builtin_inff());
