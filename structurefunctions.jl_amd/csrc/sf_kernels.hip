@@ -1452,8 +1452,14 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   int h;
   // 24-bit multiplies (full rate): every operand of a slot that is used is a small bin index;
   // values of a pair that is not added may be garbage and are selected away below
-  if constexpr (J) h = __mul24(vb0, m.jv) + __mul24(db, m.jd) + m.hoff;
-  else h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
+  if constexpr (J) {
+    // two 24-bit multiply-adds (the compiler forms two multiplies and a three-way add)
+    int hv;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(hv) : "v"(vb0), "s"(m.jv), "v"(m.hoff));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(h) : "v"(db), "s"(m.jd), "v"(hv));
+  } else {
+    h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
+  }
   if constexpr (FX) {
     // a value below a quarter bin width goes to the binary64 cell of its (distance, value) bin
     // at once (few lanes, so a branch): the fixed-point error (up to 2^-22 bin widths) would not
