@@ -253,6 +253,14 @@ DEV int val_bin64(double v, const double *E, int nE, const Slice &s) {
     g = g < 0 ? 0 : (g > nE ? nE : g);
     while (g > 0 && !(E[g - 1] < v)) --g;
     while (g < nE && E[g] < v) ++g;
+  } else if (s.v_log) {
+    // log-spaced edges from edge vk0 on: a guess from the binary32 log2 of v, then the walk,
+    // which alone decides (the count of edges below v; v <= 0 starts from edge 0)
+    float tg = __builtin_fmaf(__builtin_amdgcn_logf((float)v) - s.vl0f, s.vlinvf, (float)s.vk0 + 1.0f);
+    tg = __builtin_fminf(__builtin_fmaxf(tg, 0.0f), (float)nE);
+    g = (int)tg;
+    while (g > 0 && !(E[g - 1] < v)) --g;
+    while (g < nE && E[g] < v) ++g;
   } else {
     int lo = 0, hi = nE;
     while (lo < hi) {
