@@ -596,7 +596,7 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
   // value axis
   s.v_uniform = 0;
   s.v_negzero = 0;
-  s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vslack = 0; s.vabs = 0;
+  s.v0 = 0; s.vinv = 0; s.v0f = 0; s.vinvf = 0; s.vtc = 0; s.vslack = 0; s.vabs = 0;
   s.v_fx = 0; s.vwf = 0; s.vsf = 0; s.fxc1 = 0; s.fxc0 = 0; s.plane = 0; s.vsmall = 0; s.gz = 0;
   s.v_log = 0; s.vk0 = 0; s.vl0f = 0; s.vlinvf = 0;
   if (p.n_val > 0) {
@@ -614,6 +614,9 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
         s.vinv = (double)(1.0L / w);
         s.v0f = (float)v0;
         s.vinvf = (float)(1.0L / w);
+        // exact product of two binary32 numbers, rounded once: within 2^-24 |v0 / w| of
+        // -v0f / w, which the slack's 4u (1 + |v0 / w|) term covers
+        s.vtc = (float)(-(double)s.v0f * (double)s.vinvf);
         // slack in t units: edge non-uniformity + rounding of v0 and 1/w in binary32
         const long double u = ldexpl(1.0L, -24);
         s.vslack = round_up_f(dev + 4.0L * u * (1.0L + fabsl((long double)v0 / w)) + 4.0L * u * nv + 1e-6L +

@@ -70,6 +70,7 @@ struct Slice {
   int32_t v_negzero;  // count of -0.0 value edges (Julia isless places +0.0 above them)
   double v0, vinv;    // value guess t = (v - v0) * vinv
   float v0f, vinvf;   // binary32 value guess
+  float vtc;          // -v0f * vinvf rounded to binary32: t = fmaf(v, vinvf, vtc)
   float vslack;       // binary32 value-axis slack in units of t (edge non-uniformity + rounding)
   float vabs;         // binary32 value-axis absolute slack of searched edges (LinearBinEdges rule)
   int32_t v_fx;       // value edges uniform to 1/64 of a bin: fixed-point joint accumulation allowed

@@ -1082,10 +1082,10 @@ template <bool VU>
 DEV int val_bin32(float v, float dva, float dvb, const Slice &s, const float *E32, int nE, bool &amb,
                   float &gf) {
   if constexpr (VU) {  // uniform value edges (s.v_uniform; chosen once per column range)
-    const float t = (v - s.v0f) * s.vinvf;
-    // error of t: value bound (1/w > 0), host slack (edge non-uniformity, rounding of v0 and
-    // 1/w), and the rounding of the subtraction, product and the two offsets below (a few ulp of
-    // |t|).  Undecidable iff an integer lies in (t - eps, t + eps].  Far outside the edges
+    // t = v / w - v0 / w in one fma (its product exact, one rounding)
+    const float t = __builtin_fmaf(v, s.vinvf, s.vtc);
+    // error of t: value bound (1/w > 0), host slack (edge non-uniformity, rounding of v0, 1/w and
+    // -v0 / w), and the rounding of the fma and the two offsets below (a few ulp of |t|).  Undecidable iff an integer lies in (t - eps, t + eps].  Far outside the edges
     // (|t| >= 2^21) eps exceeds 1, so such values are undecidable too (and re-binned); the
     // float-to-int conversion saturates, so no clamp is needed.  A non-finite v is undecidable
     // without a test of its own: t = +-Inf makes eps infinite and t - eps or t + eps NaN, and a
@@ -1550,7 +1550,7 @@ DEV void queue_wave_move(const Params &P, const Smem &m, unsigned fill, int slic
   wave_lds_sync();
 }
 
-// Queue the pairs (i, jg + t) whose bit t is set in dm for the binary64 re-bin, as record
+// Queue the pairs (i, jg + step t) whose bit t is set in dm for the binary64 re-bin, as record
 // indices (the re-bin kernel maps them to the caller's indices).  Called by every active lane of
 // the wave after a group of columns, outside the pair loop: a branch there (with its global atomic
 // and lane exchange) splits the loop body into pieces the compiler cannot overlap.  Each wave owns
@@ -1561,7 +1561,7 @@ DEV void queue_wave_move(const Params &P, const Smem &m, unsigned fill, int slic
 // workgroup never wait on each other for the queue.  A group larger than a segment goes to the
 // global queue directly.
 DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int jg, int slice_id,
-                        int fields = 0) {
+                        int fields = 0, int step = 1) {
   if (__ballot(dm != 0u) == 0ull) return;  // the common case: nothing to queue in the wave
   const unsigned n = (unsigned)__popc(dm);
   unsigned tot;
@@ -1579,7 +1579,7 @@ DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int 
     while (dm) {
       const int t = __ffs((int)dm) - 1;
       dm &= dm - 1u;
-      seg[e++] = make_int2(i, jg + t);
+      seg[e++] = make_int2(i, jg + step * t);
     }
     *fillp = fill + tot;  // every active lane stores the same value
     return;
@@ -1588,7 +1588,7 @@ DEV void queue_deferred(const Params &P, const Smem &m, unsigned dm, int i, int 
   while (dm) {
     const int t = __ffs((int)dm) - 1;
     dm &= dm - 1u;
-    if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + t, fields};
+    if (g < (unsigned long long)P.pending_cap) P.pending[g] = Pending{slice_id, i, jg + step * t, fields};
     ++g;
   }
 }
@@ -1616,24 +1616,23 @@ DEV void run_chunk32_vu(const Params &P, const Slice &s, const Smem &m, const fl
   load_rec<F>(cp, 0, ra);
   PendingAdd pa{m.scr, 0.f, 0.f, 0u};
   int j = j_begin;
-  // groups of 32 columns: bit t of dm flags pair (i, jg + t) for the re-bin queue
+  // groups of 32 columns: the flags are shifted in, so bit t of dm flags pair (i, je - 1 - t)
+  // for the re-bin queue (one shift-add per pair)
   while (j < j_end) {
     const int jg = j;
     const int je = min(j_end, jg + 32);
     unsigned dm = 0u;
     for (; j + 1 < je; j += 2, cp += 2 * R) {
-      const unsigned b = 1u << (j - jg);
-      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, true, pa)) dm |= b;
-      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j + 1, rb, ra, cp + 2 * R, j + 2 < j_end, pa))
-        dm |= b << 1;
+      dm = (dm << 1) | (unsigned)pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, true, pa);
+      dm = (dm << 1) |
+           (unsigned)pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j + 1, rb, ra, cp + 2 * R, j + 2 < j_end, pa);
     }
     // an odd count only occurs in the last group (the others hold 32 columns)
     if (j < je) {
-      if (pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, false, pa))
-        dm |= 1u << (j - jg);
+      dm = (dm << 1) | (unsigned)pair32<G, D, J, OPK, MASK, FX, VU, SL>(P, s, m, ri, i, j, ra, rb, cp + R, false, pa);
       ++j;
     }
-    queue_deferred(P, m, dm, i, jg, slice_id);
+    queue_deferred(P, m, dm, i, je - 1, slice_id, 0, -1);
   }
   if constexpr (FX) {
     atomicAdd(&m.hq[pa.h], (1ull << kFxShift) | (unsigned long long)pa.q);
