@@ -1403,8 +1403,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     } else {
       dva = P.vcoef * pow_order<OPK>(P, mm);
     }
-    // both velocities tiny: the bound could underflow in binary32, so re-bin in binary64
-    amb = amb || !(umax >= 0x1p-30f);
+    // both velocities tiny: a bound of order 2 or 3 could underflow in binary32 while it still
+    // matters in units of a (tiny) bin, so re-bin in binary64.  Order 1 needs no test: a bound
+    // that underflows is below 2^-126 bin widths, far inside the slack, and a NaN fails the
+    // value-bin test by itself.
+    if constexpr (!(OPK == 1 || OPK == 6)) amb = amb || !(umax >= 0x1p-30f);
     float gf;
     vb0 = val_bin32<VU>(v, dva, ampv, s, m.vE32, P.n_val + 1, amb, gf);
     vb_ok = (unsigned)vb0 < (unsigned)P.n_val;
