@@ -887,10 +887,11 @@ DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
   asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
-  const bool above = key > lh.y;
-  bool exact_dist = (key >= lh.x) & !above;
+  // outside [lo, hi] a key is above the edge exactly when key >= lo
+  const bool ge = key >= lh.x;
+  bool exact_dist = ge & (key <= lh.y);
   if constexpr (G == kFlat) exact_dist = exact_dist | !(key > 0x1p-100f);
-  int db = above ? blo + 1 : blo;
+  int db = ge ? blo + 1 : blo;
   load_rec64<S>(next, rn);
   __builtin_amdgcn_sched_barrier(0);
   double v, w = 0.0;
@@ -1257,10 +1258,11 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // operations): the next column's record is requested only after this point, because a
     // scalar load still in flight would make the wait for the table read wait for it as well
     asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
-    // one compare with hi serves both (a NaN key fails key >= lo either way)
-    const bool above = key > lh.y;
-    amb_d = (key >= lh.x) & !above;
-    db = above ? blo + 1 : blo;
+    // two compares: a key that is not in [lo, hi] is above the edge exactly when key >= lo (a
+    // NaN key fails both: not ambiguous here, bin blo, and degenerate by the tests below)
+    const bool ge = key >= lh.x;
+    amb_d = ge & (key <= lh.y);
+    db = ge ? blo + 1 : blo;
   }
   // scalar-load variant: the request is unconditional (past the last column it reads the record
   // array's one-record tail pad) and pinned here, so that the scheduler does not sink it towards
