@@ -553,12 +553,12 @@ void build_tables(const sf_problem &p, double xmax, SliceTables &T) {
     hi[k] = round_up_f(C[k] + fabsl(C[k]) * drel + a);
   }
   std::vector<long double> ckeys(C.begin(), C.end());
-  Partition P32 = make_partition(ckeys, 1024, 6);
+  Partition P32 = make_partition(ckeys, kCell32Max, 6);
   // keep every cell's lower bound representable in binary32
   P32.elo = std::max(P32.elo, -125);
   P32.ehi = std::min(P32.ehi, 127);
   if (P32.ehi <= P32.elo) P32.ehi = P32.elo + 1;
-  const int n32 = P32.ncell();
+  const int n32 = P32.ncell();  // <= kCell32Max: at most 252 binades, M chosen to fit
   T.tab32.resize(n32);
   for (int c = 0; c < n32; ++c) {
     const float Fc = (c == 0) ? -INFINITY : (float)P32.low(c);

@@ -309,7 +309,7 @@ DEV int cell32(float key, const Slice &s);
 template <int G, int D, bool J, int OPK = 0, bool CERT = false>
 DEV int exact_cell(const Params &P, const Slice &s, const double *A, const double *B,
                    const Cell *tab, const double *dE, const double *vE, double *val,
-                   const float2 *lohi = nullptr, const int *blo_tab = nullptr, int *bins = nullptr) {
+                   const float2 *lohi = nullptr, const int2 *blo_tab = nullptr, int *bins = nullptr) {
   const int nEd = P.n_dist + 1;
   int db;
   double v;
@@ -394,7 +394,7 @@ DEV int exact_cell(const Params &P, const Slice &s, const double *A, const doubl
       const int c = cell32(key, s);
       const float2 lh = lohi[c];
       exact_dist = (key >= lh.x) & (key <= lh.y);
-      db = (key > lh.y) ? blo_tab[c] + 1 : blo_tab[c];
+      db = (key > lh.y) ? blo_tab[c].x + 1 : blo_tab[c].x;
       if (exact_dist) f.distance = sphere_distance64(A, B, f.d, P.radius);
     } else {
       f = sphere_frame64(A, B, P.radius);
@@ -525,7 +525,8 @@ struct Smem {
   unsigned *hc;
   Cell *tab;      // binary64 kernel: cell table (bx)
   float2 *lohi;   // binary32 kernel: ambiguity interval per cell ...
-  int *blo;       // ... and bin count below it (same LDS region as tab, structure of arrays: one
+  int2 *blo;      // ... and bin count below it (.x; same LDS region as tab, kCell32Max * 8 bytes
+                  // after lohi at the same stride, so that one address serves both reads; one
                   // 16-byte read of a whole cell measured 2 % slower than these 8 + 4 bytes)
   double *dE;
   double *vE;
@@ -589,7 +590,7 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.mscr = P.n_dist * m.mcop + (int)__lane_id();
   m.tab = (Cell *)(base + L.tab);
   m.lohi = (float2 *)(base + L.tab);
-  m.blo = (int *)(base + L.tab + 8 * P.ncell_max);
+  m.blo = (int2 *)(base + L.tab + 8 * kCell32Max);
   m.dE = (double *)(base + L.dE);
   m.vE = (double *)(base + L.vE);
   m.vE32 = (float *)(base + L.vE32);
@@ -764,7 +765,7 @@ DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
       const Cell e = src[c];
       m.lohi[c] = make_float2(e.lo, e.hi);
-      m.blo[c] = e.blo;
+      m.blo[c] = make_int2(e.blo, 0);
     }
   } else {
     for (int c = threadIdx.x; c < nc; c += blockDim.x) m.tab[c] = src[c];
@@ -882,7 +883,7 @@ DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A
   }
   const int c = cell32(key, s);
   const float2 lh = m.lohi[c];
-  const int blo = m.blo[c];
+  const int blo = m.blo[c].x;
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
@@ -1242,7 +1243,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   {
     const int c = cell32(key, s);
     const float2 lh = m.lohi[c];
-    const int blo = m.blo[c];
+    const int blo = m.blo[c].x;
     // the previous pair's update, unconditional (a lane with nothing pending holds its scratch
     // slot in pa.h) so that the wait below is a partial one on every path
     __builtin_amdgcn_sched_barrier(0);
@@ -1806,7 +1807,7 @@ DEV void run_chunk_strip(const Params &P, const Slice &s, const Smem &m, const f
       }
       const int c = cell32(key, s);
       const float2 lh = m.lohi[c];
-      const int blo = m.blo[c];
+      const int blo = m.blo[c].x;
       const bool amb_d = (key >= lh.x) & (key <= lh.y);
       const int db = (key > lh.y) ? blo + 1 : blo;
       const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
