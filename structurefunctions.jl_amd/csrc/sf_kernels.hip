@@ -525,7 +525,8 @@ struct Smem {
   unsigned *hc;
   Cell *tab;      // binary64 kernel: cell table (bx)
   float2 *lohi;   // binary32 kernel: ambiguity interval per cell ...
-  int2 *blo;      // ... and bin count below it (.x; same LDS region as tab, kCell32Max * 8 bytes
+  int2 *blo;      // ... and bin count below it (.x; .y holds it minus one, the 0-based bin of keys
+                  // below the interval; same LDS region as tab, kCell32Max * 8 bytes
                   // after lohi at the same stride, so that one address serves both reads; one
                   // 16-byte read of a whole cell measured 2 % slower than these 8 + 4 bytes)
   double *dE;
@@ -538,7 +539,6 @@ struct Smem {
   int nd, nslots;
   int jv, jd, jslots;  // joint slot of (d, v) in one copy: v jv + d jd; a copy holds jslots slots
   int coff;            // this lane's copy offset (joint; slots)
-  int hoff;            // coff - jd: slot of (d, v) for a 1-based distance bin d is v jv + d jd + hoff
   int scr;             // this lane's scratch slot (target of updates that add nothing)
   int nmom;     // sums per slot, moment-major: moment k of slot h at hs[k * mstride + h]
   int mstride;  // nslots + kDummySlots; slot nslots + lane is lane's scratch slot
@@ -572,7 +572,6 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.jd = JL.jd;
   m.jslots = P.n_val > 0 ? JL.slots : 0;
   m.coff = (P.n_val > 0 && P.copies == 2) ? (int)(__lane_id() >> 5) * JL.slots : 0;
-  m.hoff = m.coff - m.jd;
   m.nslots = P.n_val > 0 ? JL.slots * P.copies : P.n_dist * P.copies;
   // B-strip: field b's histogram at slots [b nslots, (b + 1) nslots), scratch slots after all
   m.mstride = m.nslots * (P.strip > 1 ? P.strip : 1) + kDummySlots;
@@ -765,7 +764,7 @@ DEV void load_tables(const Smem &m, const Params &P, const Slice &s, bool fast32
     for (int c = threadIdx.x; c < nc; c += blockDim.x) {
       const Cell e = src[c];
       m.lohi[c] = make_float2(e.lo, e.hi);
-      m.blo[c] = make_int2(e.blo, 0);
+      m.blo[c] = make_int2(e.blo, e.blo - 1);
     }
   } else {
     for (int c = threadIdx.x; c < nc; c += blockDim.x) m.tab[c] = src[c];
@@ -1224,7 +1223,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   const bool upper = !MASK || j > i;
   bool amb, amb_d;
   float key, L = 0.f, T = 0.f, tn2 = 0.f, n2 = 0.f, amp = 1.0f, umax;
-  int db;
+  int db0;  // 0-based distance bin
   // ---- distance key and certified bin
   float dx[3], du[3], d[3];
   if constexpr (G == kFlat) {
@@ -1243,7 +1242,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   {
     const int c = cell32(key, s);
     const float2 lh = m.lohi[c];
-    const int blo = m.blo[c].x;
+    const int bm1 = m.blo[c].y;  // bin count below the cell's interval, minus one
     // the previous pair's update, unconditional (a lane with nothing pending holds its scratch
     // slot in pa.h) so that the wait below is a partial one on every path
     __builtin_amdgcn_sched_barrier(0);
@@ -1258,12 +1257,12 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     // consume the table entry here (an empty asm reading it, ordered before later memory
     // operations): the next column's record is requested only after this point, because a
     // scalar load still in flight would make the wait for the table read wait for it as well
-    asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(blo) : "memory");
+    asm volatile("" ::"v"(lh.x), "v"(lh.y), "v"(bm1) : "memory");
     // two compares: a key that is not in [lo, hi] is above the edge exactly when key >= lo (a
     // NaN key fails both: not ambiguous here, bin blo, and degenerate by the tests below)
     const bool ge = key >= lh.x;
     amb_d = ge & (key <= lh.y);
-    db = ge ? blo + 1 : blo;
+    db0 = ge ? bm1 + 1 : bm1;
   }
   // scalar-load variant: the request is unconditional (past the last column it reads the record
   // array's one-record tail pad) and pinned here, so that the scheduler does not sink it towards
@@ -1275,7 +1274,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     load_rec<F>(next, 0, rn);
   }
   amb = amb_d;
-  const bool db_ok = (unsigned)(db - 1) < (unsigned)nb;
+  const bool db_ok = (unsigned)db0 < (unsigned)nb;
   bool vb_ok = true;
   float v = 0.f;
   int vb0 = 0;
@@ -1451,7 +1450,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   if constexpr (MULTI) {
     // the fused 1D outputs of a certain, in-range distance bin (whatever the joint value bin)
     const bool add1 = upper & !amb & db_ok;
-    const int h1 = add1 ? (db - 1) * m.mcop + (int)(__lane_id() & (m.mcop - 1)) : m.mscr;
+    const int h1 = add1 ? db0 * m.mcop + (int)(__lane_id() & (m.mcop - 1)) : m.mscr;
     atomicAdd(&m.mc[h1], 1u);
     double *const mp = m.ms + h1;
 #pragma unroll
@@ -1469,10 +1468,10 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
   if constexpr (J) {
     // two 24-bit multiply-adds (the compiler forms two multiplies and a three-way add)
     int hv;
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(hv) : "v"(vb0), "s"(m.jv), "v"(m.hoff));
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(h) : "v"(db), "s"(m.jd), "v"(hv));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(hv) : "v"(vb0), "s"(m.jv), "v"(m.coff));
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(h) : "v"(db0), "s"(m.jd), "v"(hv));
   } else {
-    h = (db - 1) * m.copies + (__lane_id() & (m.copies - 1));
+    h = db0 * m.copies + (__lane_id() & (m.copies - 1));
   }
   if constexpr (FX) {
     // a value below a quarter bin width goes to the binary64 cell of its (distance, value) bin
@@ -1483,7 +1482,7 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
     const int k = min(max(vb0 - s.gz, 0), kZeroBins - 1);
     const bool small = add & (__builtin_fabsf(v) < s.vsmall);
     if (small) {
-      const int z = (db - 1) * kZeroBins + k;
+      const int z = db0 * kZeroBins + k;
       atomicAdd(&m.zs[z], (double)v);
       atomicAdd(&m.zc[z], 1u);
     }
