@@ -1060,7 +1060,9 @@ int plan_build(sf_ctx *ctx, Plan &pl, int n_in, const sf_problem *sl_in, const d
   Pk.n1d = pl.n1d;
   const int nmom = Pk.nmom;
   Tuning tn = ctx->tune;
-  if (!tn.any()) {
+  // the autotuner times one-operator launches; a fused launch (more LDS, more work per pair) keeps
+  // the built-in shapes rather than a shape tuned for its main operator alone
+  if (!tn.any() && pl.n1d == 0) {
     int64_t in_pairs = 0;
     for (int s = 0; s < n_in; ++s) in_pairs += pairs_of(sl_in[s].N);
     const auto it = ctx->tuned.find(tune_key(sl_in[0], in_pairs));

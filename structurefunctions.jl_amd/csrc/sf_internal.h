@@ -233,8 +233,10 @@ inline __host__ __device__ bool sp2d(int op, int n_val) { return op == 6 && n_va
 
 inline constexpr int kDummySlots = 64;
 // largest binary32 key table (cells); the host's partition never exceeds it.  In LDS the table's
-// (lo, hi) pairs start the region and its bin counts sit kCell32Max * 8 bytes further on, at the
-// same 8-byte stride: one address serves both reads (the second takes a constant offset)
+// (lo, hi) pairs start the region and its bin counts follow at the same 8-byte stride; for joint
+// histograms they sit kCell32Max * 8 bytes on, so that one address serves both reads (the second
+// takes a constant offset), for 1D ones right after the largest table of the launch (their LDS
+// decides how many workgroups fit: 16 KB of table region cost them 12 %)
 inline constexpr int kCell32Max = 1024;
 // largest re-bin queue staged in LDS per workgroup (entries, split evenly between its waves);
 // the host shrinks it where the LDS it takes would cost a workgroup per CU
@@ -279,7 +281,7 @@ inline __host__ __device__ SmemLayout smem_layout(int n_dist, int n_val, int nce
   const int nm = n1d > 0 ? n_dist * mcopies + kDummySlots : 0;
   L.ms = o; o += 8 * nm * fused_pad(n1d);
   L.mc = o; o += 4 * ((nm + 3) & ~3);
-  L.tab = o; o += 16 * (ncell_max > kCell32Max ? ncell_max : kCell32Max);
+  L.tab = o; o += 16 * ((n_val > 0 && ncell_max < kCell32Max) ? kCell32Max : ncell_max);
   L.dE = o; o += 8 * ((n_dist + 2) & ~1);
   L.vE = o; o += 8 * ((n_val + 2) & ~1);
   L.vE32 = o; o += 4 * ((n_val + 4) & ~3);

@@ -557,6 +557,8 @@ struct Smem {
   int2 *q;
 };
 
+// JT: the launch keeps a joint histogram (P.n_val > 0), known at compile time
+template <bool JT>
 DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   const SmemLayout L = smem_layout(P.n_dist, P.n_val, P.ncell_max, stage_bytes, P.copies, P.nmom, P.fx,
                                    P.qcap, P.strip, P.jlayout, P.n1d, P.mcopies);
@@ -589,7 +591,7 @@ DEV Smem carve(char *base, const Params &P, int stage_bytes) {
   m.mscr = P.n_dist * m.mcop + (int)__lane_id();
   m.tab = (Cell *)(base + L.tab);
   m.lohi = (float2 *)(base + L.tab);
-  m.blo = (int2 *)(base + L.tab + 8 * kCell32Max);
+  m.blo = (int2 *)(base + L.tab + 8 * (JT ? kCell32Max : P.ncell_max));
   m.dE = (double *)(base + L.dE);
   m.vE = (double *)(base + L.vE);
   m.vE32 = (float *)(base + L.vE32);
@@ -1002,7 +1004,7 @@ template <int G, int D, bool J, int OPK>
 __global__ void __launch_bounds__(1024) hist64(Params P, const double *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int S = Rec<G, D>::d64;
-  const Smem m = carve(smem_raw, P, P.stage_bytes);
+  const Smem m = carve<J>(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
   const int B = blockDim.x;
   int cur = -1;
@@ -1669,7 +1671,7 @@ template <int G, int D, bool J, int OPK, bool SLOAD, bool FX>
 __global__ void __launch_bounds__(1024, (FX && SLOAD) ? 6 : 1) hist32(Params P, const float *__restrict__ rec) {
   extern __shared__ __align__(16) char smem_raw[];
   constexpr int F = Rec<G, D>::f32;
-  const Smem m = carve(smem_raw, P, P.stage_bytes);
+  const Smem m = carve<J>(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
   const int B = blockDim.x;
   int cur = -1;
@@ -1921,7 +1923,7 @@ __global__ void __launch_bounds__(1024) hist32s(Params P, const float *__restric
   extern __shared__ __align__(16) char smem_raw[];
   using SR = StripRec<G, D>;
   constexpr int F = Rec<G, D>::f32;
-  const Smem m = carve(smem_raw, P, P.stage_bytes);
+  const Smem m = carve<false>(smem_raw, P, P.stage_bytes);
   __shared__ int s_item;
   const int B = blockDim.x;
   int cur = -1, nf = 1;
