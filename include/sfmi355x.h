@@ -68,7 +68,9 @@ extern "C" {
                                  Runs as six joint histograms, one per invariant.                 */
 
 /* precision */
-#define SF_PREC_FP64 0        /* plain binary64 pair arithmetic: the drop-in (counts bit-exact)  */
+#define SF_PREC_FP64 0        /* binary64 pair arithmetic in the reference's order, bins certified
+                                 from the exact keys and values (the reference's rules near every
+                                 decision point): the drop-in (counts bit-exact)                 */
 #define SF_PREC_MIXED 1       /* binary32 pair arithmetic, binary64 re-binning of pairs near any
                                  bin edge (counts bit-exact), binary64 accumulation              */
 #define SF_PREC_MIXED_FIXED 2 /* as SF_PREC_MIXED, but a joint histogram whose value edges are
