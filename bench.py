@@ -45,7 +45,7 @@ TABLE_FLOPS = {("cfg1", "L2"): 28,
                # the six single-pass invariants from one pair evaluation: flat 2D 40; sphere: the
                # common spherical part 43, du_L 8, du_T 8, |du|^2 and the four moments 10
                ("cfg2", "single pass"): 40, ("cfg4", "single pass"): 69,
-               # fused outputs (one pair evaluation, sf_hist_multi_device), SURVEY-style counts:
+               # fused outputs (one pair evaluation, sf_hist_multi_device), counted the same way:
                # flat 2D all six projected operators 40; sphere four 1D + joint du_L 76
                ("cfg2", "fused L1 T1 L2 T2 L3 T3"): 40, ("cfg4", "fused L2 T2 L3 T3 + joint L1"): 76}
 # (configuration, operator) rows also timed with the binary64 drop-in kernel (against the FP64 peak)
