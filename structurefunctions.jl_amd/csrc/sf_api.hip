@@ -37,6 +37,7 @@ hipError_t launch_merge_checked(const double *src_s, const unsigned long long *s
                                 const unsigned long long *pcount, int64_t cap, int *flag,
                                 hipStream_t st);
 hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st);
+hipError_t launch_zero_multi(int count, void *const *p, const int64_t *bytes, hipStream_t st);
 hipError_t launch_derive_sp(double *s, unsigned long long *c, int64_t nbins, hipStream_t st);
 hipError_t launch_eval_math(int fn, int64_t n, const double *a, const double *b, double *out,
                             hipStream_t st);
@@ -1351,17 +1352,23 @@ int plan_launch(Plan &pl, bool use_mixed, double *out_s, unsigned long long *out
   const int64_t ncells = (int64_t)pl.nout * pl.ncell;
   HIPCHK(launch_prep(pl.G, pl.D, use_mixed, pl.geom_code, pl.prep.p, pl.n, pl.max_n, pl.rec64.p,
                      pl.rec32.p, st));
-  HIPCHK(launch_zero(pl.acc_s.p, sizeof(double) * ncells, st));
-  HIPCHK(launch_zero(pl.acc_c.p, sizeof(unsigned long long) * ncells, st));
-  HIPCHK(launch_zero(pl.counter.p, sizeof(int) * 4, st));
-  HIPCHK(launch_zero(pl.pcount.p, sizeof(unsigned long long) * 4, st));
-  HIPCHK(launch_zero(pl.flag.p, sizeof(int) * 4, st));
   const int64_t mcells = (int64_t)pl.n1d * pl.nout * pl.P.n_dist;
-  if (pl.n1d) {
-    if (!use_mixed)  // fused outputs exist in the binary32 kernel only
-      return ctx->fail(SF_E_UNSUPPORTED, "internal: fused outputs launched on the binary64 kernel");
-    HIPCHK(launch_zero(pl.msums.p, sizeof(double) * mcells, st));
-    HIPCHK(launch_zero(pl.mcounts.p, sizeof(unsigned long long) * mcells, st));
+  if (pl.n1d && !use_mixed)  // fused outputs exist in the binary32 kernel only
+    return ctx->fail(SF_E_UNSUPPORTED, "internal: fused outputs launched on the binary64 kernel");
+  {
+    // accumulators, the work counter, the re-bin queue count and the overflow flag: one launch
+    void *zp[kZeroSetMax] = {pl.acc_s.p, pl.acc_c.p, pl.counter.p, pl.pcount.p, pl.flag.p};
+    int64_t zb[kZeroSetMax] = {(int64_t)sizeof(double) * ncells, (int64_t)sizeof(unsigned long long) * ncells,
+                               (int64_t)sizeof(int) * 4, (int64_t)sizeof(unsigned long long) * 4,
+                               (int64_t)sizeof(int) * 4};
+    int nz = 5;
+    if (pl.n1d) {
+      zp[nz] = pl.msums.p;
+      zb[nz++] = (int64_t)sizeof(double) * mcells;
+      zp[nz] = pl.mcounts.p;
+      zb[nz++] = (int64_t)sizeof(unsigned long long) * mcells;
+    }
+    HIPCHK(launch_zero_multi(nz, zp, zb, st));
   }
   const bool use_sload = use_mixed && pl.sload;
   const int stage = (use_sload || !use_mixed) ? 0 : pl.tj * pl.F32 * 4;

@@ -232,6 +232,7 @@ inline __host__ __device__ JointLayout joint_layout(int n_dist, int n_val, int l
 inline __host__ __device__ bool sp2d(int op, int n_val) { return op == 6 && n_val > 0; }
 
 inline constexpr int kDummySlots = 64;
+inline constexpr int kZeroSetMax = 8;  // buffers cleared by one zero_multi launch
 // largest binary32 key table (cells); the host's partition never exceeds it.  In LDS the table's
 // (lo, hi) pairs start the region and its bin counts follow at the same 8-byte stride; for joint
 // histograms they sit kCell32Max * 8 bytes on, so that one address serves both reads (the second

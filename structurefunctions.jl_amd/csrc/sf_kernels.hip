@@ -2244,6 +2244,18 @@ __global__ void zero_i32(int *p, int64_t n) {
   if (k < n) p[k] = 0;
 }
 
+// Several buffers cleared by one launch (one row of blocks per buffer): a call's accumulators,
+// counters and flags, one kernel instead of one per buffer.
+struct ZeroSet {
+  int *p[kZeroSetMax];
+  int64_t n[kZeroSetMax];  // 32-bit words
+};
+__global__ void zero_multi(ZeroSet z) {
+  const int b = blockIdx.y;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < z.n[b]) z.p[b][k] = 0;
+}
+
 }  // namespace sfk
 
 // =============================================================================================
@@ -2453,6 +2465,20 @@ hipError_t launch_zero(void *p, int64_t bytes, hipStream_t st) {
   const int64_t n = bytes / 4;
   if (n <= 0) return hipSuccess;
   zero_i32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>((int *)p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_multi(int count, void *const *p, const int64_t *bytes, hipStream_t st) {
+  if (count <= 0 || count > kZeroSetMax) return hipErrorInvalidValue;
+  ZeroSet z{};
+  int64_t nmax = 0;
+  for (int b = 0; b < count; ++b) {
+    z.p[b] = (int *)p[b];
+    z.n[b] = bytes[b] / 4;
+    nmax = nmax > z.n[b] ? nmax : z.n[b];
+  }
+  if (nmax <= 0) return hipSuccess;
+  zero_multi<<<dim3((unsigned)((nmax + 255) / 256), (unsigned)count), 256, 0, st>>>(z);
   return hipGetLastError();
 }
 
