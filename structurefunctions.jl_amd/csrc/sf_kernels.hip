@@ -863,7 +863,7 @@ struct Pending64 {
   double v, w;  // value (single pass: du_L) and |du|^2 (single pass)
 };
 
-template <int G, int D, bool J, int OPK, bool MASK>
+template <int G, int D, bool J, int OPK, bool MASK, bool VU>
 DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A, int i, int j,
                  const double *B, double *rn, const double *next, Pending64 &pa) {
   constexpr int S = Rec<G, D>::d64;
@@ -946,7 +946,7 @@ DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A
   int vb = 1;
   bool amb_v = false;
   if constexpr (J) {
-    if (s.v_uniform) {
+    if constexpr (VU) {  // s.v_uniform, chosen once per column range
       const float t = (float)(v - s.v0) * s.vinvf;
       const float eps = __builtin_fmaf(9.6e-7f, __builtin_fabsf(t), s.vslack);
       const float gh = __builtin_floorf(t + eps);
@@ -982,9 +982,9 @@ DEV void pair64p(const Params &P, const Slice &s, const Smem &m, const double *A
 
 // Columns j0 .. j1 - 1 of one row with pair64p; two record buffers alternate (the request past
 // the last column reads at most one record beyond the slice, as in run_cols64).
-template <int G, int D, bool J, int OPK, bool MASK>
-DEV void run_cols64p(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
-                     const double *__restrict__ rec) {
+template <int G, int D, bool J, int OPK, bool MASK, bool VU>
+DEV void run_cols64pv(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
+                      const double *__restrict__ rec) {
   constexpr int S = Rec<G, D>::d64;
   const double *col = rec + (int64_t)s.pt_off * S;
   double ra[S], rb[S];
@@ -992,12 +992,20 @@ DEV void run_cols64p(const Params &P, const Slice &s, const Smem &m, const doubl
   Pending64 pa{m.scr, 0.0, 0.0};
   int j = j0;
   for (; j + 1 < j1; j += 2) {
-    pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
-    pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j + 1, rb, ra, col + (int64_t)(j + 2) * S, pa);
+    pair64p<G, D, J, OPK, MASK, VU>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+    pair64p<G, D, J, OPK, MASK, VU>(P, s, m, ri, i, j + 1, rb, ra, col + (int64_t)(j + 2) * S, pa);
   }
-  if (j < j1) pair64p<G, D, J, OPK, MASK>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
+  if (j < j1) pair64p<G, D, J, OPK, MASK, VU>(P, s, m, ri, i, j, ra, rb, col + (int64_t)(j + 1) * S, pa);
   if constexpr (OPK == 3) hist_add_sp(m, pa.h, pa.v, pa.w);
   else hist_add(m, pa.h, pa.v);
+}
+
+// the value-bin lookup (uniform or exact) is chosen once per column range, not per pair
+template <int G, int D, bool J, int OPK, bool MASK>
+DEV void run_cols64p(const Params &P, const Slice &s, const Smem &m, const double *ri, int i, int j0, int j1,
+                     const double *__restrict__ rec) {
+  if (J && s.v_uniform) run_cols64pv<G, D, J, OPK, MASK, true>(P, s, m, ri, i, j0, j1, rec);
+  else run_cols64pv<G, D, J, OPK, MASK, false>(P, s, m, ri, i, j0, j1, rec);
 }
 
 template <int G, int D, bool J, int OPK>
