@@ -1411,7 +1411,15 @@ DEV bool pair32(const Params &P, const Slice &s, const Smem &m, const float *ri,
       dva = s.plane < 3 ? 0x1p-18f * u2 : 0x1.8p-18f * u2 * mm;
       if constexpr (VU) dva *= s.vinvf;
     } else if constexpr (VU) {
-      dva = (P.vcoef * s.vinvf) * pow_order<OPK>(P, mm);
+      if constexpr (OPK == 1 && !TIGHT) {
+        // K (|u_i| + |u_j|) as one fma on the row's K |u_i| (loop-invariant): the same bound to
+        // an ulp, against which the coefficient keeps a factor of 5 (umax itself is then unused)
+        constexpr int UI = (G == kFlat) ? 4 * D : 12;
+        const float kv = P.vcoef * s.vinvf;
+        dva = __builtin_fmaf(kv, rj[UI], kv * ri[UI]);
+      } else {
+        dva = (P.vcoef * s.vinvf) * pow_order<OPK>(P, mm);
+      }
     } else {
       dva = P.vcoef * pow_order<OPK>(P, mm);
     }
